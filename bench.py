@@ -1,0 +1,155 @@
+"""Benchmark: FP64 RK4 ray-steps/s of the many-ray integrator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (a "step" = one pass of the hot path over one batch): the
+seamount-jet case, a 1024 x 1024 grid at 200 m with a Gaussian seamount and a
+zonal jet, 65,536 rays, 5000 RK4 steps of 1 s each, full step-major
+trajectories written to HBM.  Initial conditions, field tables and output
+buffers are resident on the GPU before the timed region; the timed region is
+K back-to-back launches of the trace kernel.
+
+Printed: one JSON line with the throughput, the dominant kernel's roofline
+(achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
+its own event-timed duration), and the CPU oracle timed on a bounded sample
+of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FLOP_PER_RAY_STEP = 745          # 265 basic FP64 ops + 24 transcendentals x 20
+PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 (vector and matrix)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_kernel_seamount_jet_traffic.json")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--rays", type=int, default=65536)
+    p.add_argument("--n-steps", type=int, default=5000, help="RK4 steps per ray")
+    p.add_argument("--math", default="fast", choices=["fast", "literal"])
+    p.add_argument("--block", type=int, default=0)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-rays", type=int, default=16384)
+    return p.parse_args()
+
+
+def cpu_baseline(cfg, n_rays):
+    """The CPU oracle (OpenMP over rays) on the first n_rays rays."""
+    import numpy as np
+    import oracle
+    oracle.build()
+    b = oracle.Bathymetry.grid(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
+    c = oracle.Current.grid(cfg["current"]["x"], cfg["current"]["y"], cfg["current"]["u"],
+                            cfg["current"]["v"])
+    sl = slice(0, n_rays)
+    t0 = time.perf_counter()
+    r = oracle.trace_many(b, c, cfg["x0"][sl], cfg["y0"][sl], cfg["kx0"][sl], cfg["ky0"][sl],
+                          cfg["t0"], cfg["t_end"], cfg["dt"], trajectories=False)
+    el = time.perf_counter() - t0
+    steps = int((r["len"].astype(np.int64) - 1).sum())
+    return {"value": steps / el, "unit": "ray-steps/s", "cores": oracle.max_threads(),
+            "kind": "port",
+            "sample": f"first {n_rays} rays of the same workload, all steps "
+                      f"({steps} ray-steps in {el:.2f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(rays shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mantaray_amd import core, workloads
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    cfg = workloads.seamount_jet(n_rays=args.rays, n_steps=args.n_steps)
+    b, c = workloads.make_fields(cfg)
+    b.upload(0)
+    c.upload(0)
+    ic = [torch.tensor(cfg[k], dtype=torch.float64, device=dev) for k in ("x0", "y0", "kx0", "ky0")]
+    n = core.num_steps(cfg["t0"], cfg["t_end"], cfg["dt"])
+    nr = args.rays
+    out = {"t": torch.empty(n + 1, dtype=torch.float64, device=dev),
+           "len": torch.empty(nr, dtype=torch.int32, device=dev),
+           "end": torch.empty((nr, 4), dtype=torch.float64, device=dev)}
+    for k in ("x", "y", "kx", "ky"):
+        out[k] = torch.empty((n + 1, nr), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def launch():
+        core.trace_many_device(b, c, *ic, cfg["t0"], cfg["t_end"], cfg["dt"], math=args.math,
+                               block_size=args.block, out=out, stream=stream)
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    ray_steps = int((out["len"].to(torch.int64) - 1).sum().item())
+
+    # timed region: K launches bracketed by synchronisation; each launch also
+    # bracketed by events on the launch stream for the kernel's own duration
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        launch()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
+
+    value = ray_steps * args.steps / wall
+    achieved = ray_steps * FLOP_PER_RAY_STEP / (kern_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    res = {
+        "metric": "FP64 RK ray-steps/s on 1 MI355X vs host CPU",
+        "value": value,
+        "unit": "ray-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "seamount_jet", "grid": "1024x1024 @ 200 m", "rays": nr,
+                   "rk4_steps": n, "dt_s": cfg["dt"], "ray_steps_per_launch": ray_steps,
+                   "outputs": "full step-major trajectories (x, y, kx, ky) in HBM",
+                   "math": args.math, "parallelism": "rays, one per lane"},
+        "roofline": {"bound": "mfma", "roof": "FP64 vector/matrix peak",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel_ms": kern_ms,
+                     "work": f"{FLOP_PER_RAY_STEP} flop/ray-step x {ray_steps} ray-steps"},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_rays, nr))
+        res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,710 @@
+// mr_capi.cpp -- the C ABI of libmantaray_hip.so (include/mantaray_capi.h):
+// field handles, their per-device tables, and the integration entry points.
+//
+// Citations are to files of mines-oceanography/mantaray 0.1.0.
+#include "mantaray_capi.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mr_kernels.h"
+#include "mr_netcdf3.h"
+
+using namespace mr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define MR_HIP_TRY(expr)                                                             \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(e_ == hipErrorOutOfMemory ? MR_ERR_OUT_OF_MEMORY : MR_ERR_DEVICE, \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+bool is_bathy(int32_t k) { return k >= MR_BATHY_CONSTANT && k <= MR_BATHY_GRID; }
+bool is_current(int32_t k) { return k == MR_CURRENT_CONSTANT || k == MR_CURRENT_GRID; }
+
+// device allocations of one field on one device
+struct DevTables {
+    std::vector<void *> allocs;
+    void *view = nullptr;          // BathyView or CurrentView in device memory
+    ~DevTables()
+    {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+};
+
+// restores the caller's current device on scope exit
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        ok = dev < 0 || hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <class T>
+int dev_upload(DevTables &t, const T *host, size_t n, T **out)
+{
+    *out = nullptr;
+    if (n == 0) return MR_OK;
+    void *p = nullptr;
+    MR_HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    t.allocs.push_back(p);
+    MR_HIP_TRY(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<T *>(p);
+    return MR_OK;
+}
+
+template <class T>
+int dev_alloc(DevTables &t, size_t n, T **out)
+{
+    *out = nullptr;
+    if (n == 0) return MR_OK;
+    void *p = nullptr;
+    MR_HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    t.allocs.push_back(p);
+    *out = static_cast<T *>(p);
+    return MR_OK;
+}
+
+// first cell (row-major, stride nx-1) whose largest corner index
+// nx*(j+1) + i + 1 is past an array of len values
+int64_t first_oob_cell(int64_t nx, int64_t ny, int64_t len)
+{
+    const int64_t ncx = nx - 1;
+    for (int64_t j = 0; j < ny - 1; j++) {
+        const int64_t c0 = nx * (j + 1) + 1;          // corner C of cell (0, j)
+        if (c0 >= len) return j * ncx;
+        if (c0 + (ncx - 1) >= len) return j * ncx + (len - c0);
+    }
+    return ncx * (ny - 1);
+}
+
+}  // namespace
+
+struct mr_field {
+    int32_t kind = 0;
+    float h = 0, h0 = 0, x0 = 0, y0 = 0, dhdx = 0, dhdy = 0;
+    double u = 0, v = 0;
+    int64_t an = 0;
+    std::vector<float> arr;
+    int64_t nx = 0, ny = 0;
+    std::vector<float> xf, yf;      // bathymetry axes (f32)
+    std::vector<double> xd, yd;     // current axes (f64)
+    std::vector<double> a, b;       // depth, or u and v
+    std::mutex mu;
+    std::map<int, std::unique_ptr<DevTables>> dev;
+};
+
+namespace {
+
+int validate_axes_f32(const std::vector<float> &x, const char *name)
+{
+    if (x.size() < 2)
+        return fail(MR_ERR_UNSUPPORTED,
+                    std::string("axis '") + name +
+                        "' has fewer than 2 points (the reference panics on every query)");
+    for (float v : x)
+        if (!std::isfinite(v))
+            return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' is not finite");
+    if (x[1] == x[0])
+        return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' has zero spacing");
+    if (x.size() >= (1u << 24))
+        return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' is too long");
+    return MR_OK;
+}
+
+int validate_axes_f64(const std::vector<double> &x, const char *name)
+{
+    if (x.size() < 2)
+        return fail(MR_ERR_UNSUPPORTED,
+                    std::string("axis '") + name +
+                        "' has fewer than 2 points (the reference panics on every query)");
+    for (double v : x)
+        if (!std::isfinite(v))
+            return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' is not finite");
+    if (x[1] == x[0])
+        return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' has zero spacing");
+    if (x.size() >= (1u << 24))
+        return fail(MR_ERR_UNSUPPORTED, std::string("axis '") + name + "' is too long");
+    return MR_OK;
+}
+
+std::vector<float2> axis_pairs(const std::vector<float> &a)
+{
+    std::vector<float2> p(a.size() - 1);
+    for (size_t i = 0; i + 1 < a.size(); i++) p[i] = make_float2(a[i], a[i + 1]);
+    return p;
+}
+
+int build_bathy(const mr_field &f, DevTables &t)
+{
+    BathyView v;
+    std::memset(&v, 0, sizeof v);
+    v.kind = f.kind;
+    if (f.kind == MR_BATHY_CONSTANT) {
+        v.h = f.h;
+    } else if (f.kind == MR_BATHY_SLOPE) {
+        v.h0 = f.h0; v.x0 = f.x0; v.y0 = f.y0; v.dhdx = f.dhdx; v.dhdy = f.dhdy;
+    } else if (f.kind == MR_BATHY_ARRAY) {
+        float *d = nullptr;
+        int rc = dev_upload(t, f.arr.data(), f.arr.size(), &d);
+        if (rc) return rc;
+        v.arr = d;
+        v.an = f.an;
+    } else {
+        v.nx = (int)f.nx;
+        v.ny = (int)f.ny;
+        v.ax0 = f.xf[0];
+        v.ay0 = f.yf[0];
+        v.nxm1 = (float)(f.nx - 1);                 // (array.len() - 1) as f32
+        v.nym1 = (float)(f.ny - 1);
+        v.nxm2 = (float)(f.nx - 2);
+        v.nym2 = (float)(f.ny - 2);
+        v.inv_sx = 1.0 / (double)std::fabs(f.xf[1] - f.xf[0]);   // spacing in f32
+        v.inv_sy = 1.0 / (double)std::fabs(f.yf[1] - f.yf[0]);
+        std::vector<float2> xp = axis_pairs(f.xf), yp = axis_pairs(f.yf);
+        float2 *dxp, *dyp;
+        double *draw;
+        int rc = dev_upload(t, xp.data(), xp.size(), &dxp);
+        if (!rc) rc = dev_upload(t, yp.data(), yp.size(), &dyp);
+        if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &draw);
+        if (rc) return rc;
+        const int64_t ncell = (f.nx - 1) * (f.ny - 1);
+        BCell *cells;
+        rc = dev_alloc(t, (size_t)ncell, &cells);
+        if (rc) return rc;
+        BathyBuildArgs ba;
+        ba.nx = f.nx; ba.ny = f.ny; ba.xp = dxp; ba.yp = dyp; ba.raw = draw;
+        ba.oob_from = first_oob_cell(f.nx, f.ny, (int64_t)f.a.size());
+        ba.xspace = (double)f.xf[1] - (double)f.xf[0];   // cartesian_netcdf3.rs:119-120
+        ba.yspace = (double)f.yf[1] - (double)f.yf[0];
+        ba.cells = cells;
+        MR_HIP_TRY(launch_build_bathy(ba, 0));
+        v.xp = dxp; v.yp = dyp; v.cells = cells; v.oob_from = ba.oob_from; v.raw = draw;
+    }
+    BathyView *dv;
+    int rc = dev_upload(t, &v, 1, &dv);
+    if (rc) return rc;
+    MR_HIP_TRY(hipDeviceSynchronize());
+    t.view = dv;
+    return MR_OK;
+}
+
+int build_current(const mr_field &f, DevTables &t)
+{
+    CurrentView v;
+    std::memset(&v, 0, sizeof v);
+    v.kind = f.kind == MR_CURRENT_CONSTANT ? CK_CONSTANT : CK_GRID;
+    if (f.kind == MR_CURRENT_CONSTANT) {
+        v.u = f.u; v.v = f.v;
+    } else {
+        v.nx = (int)f.nx;
+        v.ny = (int)f.ny;
+        v.cx0 = f.xd[0];
+        v.cy0 = f.yd[0];
+        v.sx = std::fabs(f.xd[1] - f.xd[0]);      // cartesian_current.rs:242
+        v.sy = std::fabs(f.yd[1] - f.yd[0]);
+        v.inv_sx = 1.0 / v.sx;
+        v.inv_sy = 1.0 / v.sy;
+        v.nxm1 = (double)(f.nx - 1);
+        v.nym1 = (double)(f.ny - 1);
+        v.nxm2 = (double)(f.nx - 2);
+        v.nym2 = (double)(f.ny - 2);
+        std::vector<float> x32(f.xd.size()), y32(f.yd.size());
+        for (size_t i = 0; i < x32.size(); i++) x32[i] = (float)f.xd[i];   // :375-378
+        for (size_t i = 0; i < y32.size(); i++) y32[i] = (float)f.yd[i];
+        // uniform f32 axes: one change of basis for every cell
+        bool uniform = true;
+        const float dx0 = x32[1] - x32[0], dy0 = y32[1] - y32[0];
+        for (size_t i = 0; i + 1 < x32.size() && uniform; i++) uniform = (x32[i + 1] - x32[i]) == dx0;
+        for (size_t i = 0; i + 1 < y32.size() && uniform; i++) uniform = (y32[i + 1] - y32[i]) == dy0;
+        if (uniform) {
+            const float btx = x32[0] - x32[0], bty = dy0, dtx = dx0, dty = y32[0] - y32[0];
+            const float det = (btx * dty) - (dtx * bty);       // interpolator.rs:64
+            if (det == 0.0f) {
+                uniform = false;
+            } else {
+                v.m01 = -(dtx / det);
+                v.m10 = -(bty / det);
+            }
+        }
+        std::vector<float2> xp = axis_pairs(x32), yp = axis_pairs(y32);
+        float2 *dxp, *dyp;
+        double *du, *dv;
+        int rc = dev_upload(t, xp.data(), xp.size(), &dxp);
+        if (!rc) rc = dev_upload(t, yp.data(), yp.size(), &dyp);
+        if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &du);
+        if (!rc) rc = dev_upload(t, f.b.data(), f.b.size(), &dv);
+        if (rc) return rc;
+        const int64_t ncell = (f.nx - 1) * (f.ny - 1);
+        CCell *cells;
+        float2 *mtab = nullptr;
+        rc = dev_alloc(t, (size_t)ncell, &cells);
+        if (!rc && !uniform) rc = dev_alloc(t, (size_t)ncell, &mtab);
+        if (rc) return rc;
+        CurrentBuildArgs ca;
+        ca.nx = f.nx; ca.ny = f.ny; ca.xp = dxp; ca.yp = dyp; ca.u = du; ca.v = dv;
+        ca.oob_from = std::min(first_oob_cell(f.nx, f.ny, (int64_t)f.a.size()),
+                               first_oob_cell(f.nx, f.ny, (int64_t)f.b.size()));
+        ca.xspace = f.xd[1] - f.xd[0];               // cartesian_current.rs:512-513
+        ca.yspace = f.yd[1] - f.yd[0];
+        ca.cells = cells;
+        ca.mtab = mtab;
+        MR_HIP_TRY(launch_build_current(ca, 0));
+        v.xp = dxp; v.yp = dyp; v.cells = cells; v.oob_from = ca.oob_from; v.mtab = mtab;
+        v.raw_u = du; v.raw_v = dv;
+    }
+    CurrentView *dview;
+    int rc = dev_upload(t, &v, 1, &dview);
+    if (rc) return rc;
+    MR_HIP_TRY(hipDeviceSynchronize());
+    t.view = dview;
+    return MR_OK;
+}
+
+// the device view of a field on the current device, built on first use
+int field_view(const mr_field *cf, void **view)
+{
+    mr_field *f = const_cast<mr_field *>(cf);
+    int dev = 0;
+    MR_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(f->mu);
+    auto it = f->dev.find(dev);
+    if (it != f->dev.end()) {
+        *view = it->second->view;
+        return MR_OK;
+    }
+    std::unique_ptr<DevTables> t(new DevTables);
+    int rc = is_bathy(f->kind) ? build_bathy(*f, *t) : build_current(*f, *t);
+    if (rc) return rc;
+    *view = t->view;
+    f->dev[dev] = std::move(t);
+    return MR_OK;
+}
+
+int check_pair(const mr_field *b, const mr_field *c)
+{
+    if (!b || !c) return fail(MR_ERR_INVALID_ARGUMENT, "null field handle");
+    if (!is_bathy(b->kind)) return fail(MR_ERR_INVALID_ARGUMENT, "first field is not a bathymetry");
+    if (!is_current(c->kind)) return fail(MR_ERR_INVALID_ARGUMENT, "second field is not a current");
+    return MR_OK;
+}
+
+int resolve_options(const mr_options *opt, mr_options &o)
+{
+    mr_options_init(&o);
+    if (opt) {
+        if (opt->struct_size < (int32_t)sizeof(mr_options))
+            return fail(MR_ERR_INVALID_ARGUMENT, "mr_options.struct_size too small");
+        o = *opt;
+    }
+    if (o.precision != MR_PREC_REFERENCE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "unsupported precision mode");
+    if (o.math != MR_MATH_FAST && o.math != MR_MATH_LITERAL)
+        return fail(MR_ERR_INVALID_ARGUMENT, "unknown math mode");
+    if (o.solout != MR_SOLOUT_STATE && o.solout != MR_SOLOUT_NEXT_DERIV)
+        return fail(MR_ERR_INVALID_ARGUMENT, "unknown solout variant");
+    if (o.memory != MR_MEM_HOST && o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "unknown memory kind");
+    if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64))
+        return fail(MR_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+    return MR_OK;
+}
+
+int bathy_kind_of(const mr_field *b) { return b->kind; }   // MR_BATHY_* == BK_*
+int current_kind_of(const mr_field *c) { return c->kind == MR_CURRENT_CONSTANT ? CK_CONSTANT : CK_GRID; }
+
+int tuned_block(int64_t n_rays)
+{
+    // one wave per workgroup until every SIMD of the device has a wave
+    return n_rays >= 256 * 1024 ? 256 : 64;
+}
+
+// the trace itself, all pointers device pointers, enqueued on stream
+int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
+                 const double *x0, const double *y0, const double *kx0, const double *ky0,
+                 double t0, int64_t n, double dt, const mr_options &o, hipStream_t stream,
+                 double *t_out, double *x_out, double *y_out, double *kx_out, double *ky_out,
+                 int32_t *len_out, double *end_out)
+{
+    void *bv, *cv;
+    int rc = field_view(bathy, &bv);
+    if (!rc) rc = field_view(current, &cv);
+    if (rc) return rc;
+    if (t_out) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
+    TraceArgs a;
+    a.B = bv;
+    a.C = cv;
+    a.n_rays = n_rays;
+    a.n_steps = n;
+    a.dt = dt;
+    a.half = dt / 2.0;                                 // Rk4::new half_step
+    a.sixth = dt / 6.0;                                // Rk4::step (step_size / 6)
+    a.x0 = x0; a.y0 = y0; a.kx0 = kx0; a.ky0 = ky0;
+    a.xo = x_out; a.yo = y_out; a.kxo = kx_out; a.kyo = ky_out;
+    a.len_out = len_out;
+    a.end_out = end_out;
+    a.solout = o.solout;
+    const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
+                            stream));
+    return MR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mr_options_init(mr_options *opt)
+{
+    if (!opt) return;
+    std::memset(opt, 0, sizeof *opt);
+    opt->struct_size = (int32_t)sizeof(mr_options);
+    opt->precision = MR_PREC_REFERENCE;
+    opt->math = MR_MATH_FAST;
+    opt->solout = MR_SOLOUT_STATE;
+    opt->memory = MR_MEM_HOST;
+    opt->device = -1;
+    opt->block_size = 0;
+    opt->stream = nullptr;
+}
+
+const char *mr_last_error(void) { return g_err.c_str(); }
+
+int mr_version(int32_t *major, int32_t *minor, int32_t *patch)
+{
+    if (major) *major = MR_VERSION_MAJOR;
+    if (minor) *minor = MR_VERSION_MINOR;
+    if (patch) *patch = MR_VERSION_PATCH;
+    return MR_OK;
+}
+
+int mr_bathy_from_arrays(int64_t nx, int64_t ny, const float *x, const float *y,
+                         int64_t depth_len, const double *depth, mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (nx < 0 || ny < 0 || depth_len < 0 || (nx && !x) || (ny && !y) || (depth_len && !depth))
+        return fail(MR_ERR_INVALID_ARGUMENT, "invalid array arguments");
+    std::unique_ptr<mr_field> f(new mr_field);
+    f->kind = MR_BATHY_GRID;
+    f->nx = nx; f->ny = ny;
+    f->xf.assign(x, x + nx);
+    f->yf.assign(y, y + ny);
+    f->a.assign(depth, depth + depth_len);
+    int rc = validate_axes_f32(f->xf, "x");
+    if (!rc) rc = validate_axes_f32(f->yf, "y");
+    if (rc) return rc;
+    *out = f.release();
+    return MR_OK;
+}
+
+int mr_current_from_arrays(int64_t nx, int64_t ny, const double *x, const double *y,
+                           int64_t u_len, const double *u, int64_t v_len, const double *v,
+                           mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (nx < 0 || ny < 0 || u_len < 0 || v_len < 0 || (nx && !x) || (ny && !y) ||
+        (u_len && !u) || (v_len && !v))
+        return fail(MR_ERR_INVALID_ARGUMENT, "invalid array arguments");
+    std::unique_ptr<mr_field> f(new mr_field);
+    f->kind = MR_CURRENT_GRID;
+    f->nx = nx; f->ny = ny;
+    f->xd.assign(x, x + nx);
+    f->yd.assign(y, y + ny);
+    f->a.assign(u, u + u_len);
+    f->b.assign(v, v + v_len);
+    int rc = validate_axes_f64(f->xd, "x");
+    if (!rc) rc = validate_axes_f64(f->yd, "y");
+    if (rc) return rc;
+    *out = f.release();
+    return MR_OK;
+}
+
+int mr_bathy_open_nc3(const char *path, const char *xname, const char *yname,
+                      const char *depth_name, mr_field **out)
+{
+    if (!out || !path || !xname || !yname || !depth_name)
+        return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    NcFile nc;
+    std::string msg;
+    int rc = nc3_open(path, nc, msg);
+    if (rc) return fail(rc == -2 ? MR_ERR_IO : MR_ERR_FORMAT, msg);
+    std::vector<float> x, y;
+    std::vector<double> d;
+    // cartesian_netcdf3.rs:170-252: axes promoted to f32, depth to f64
+    if (nc3_read_f32(nc, xname, x, msg) || nc3_read_f32(nc, yname, y, msg) ||
+        nc3_read_f64(nc, depth_name, d, msg))
+        return fail(MR_ERR_FORMAT, msg);
+    return mr_bathy_from_arrays((int64_t)x.size(), (int64_t)y.size(), x.data(), y.data(),
+                                (int64_t)d.size(), d.data(), out);
+}
+
+int mr_current_open_nc3(const char *path, const char *xname, const char *yname,
+                        const char *uname, const char *vname, mr_field **out)
+{
+    if (!out || !path || !xname || !yname || !uname || !vname)
+        return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    NcFile nc;
+    std::string msg;
+    int rc = nc3_open(path, nc, msg);
+    if (rc) return fail(rc == -2 ? MR_ERR_IO : MR_ERR_FORMAT, msg);
+    std::vector<double> x, y, u, v;
+    // cartesian_current.rs:67-205: everything promoted to f64
+    if (nc3_read_f64(nc, xname, x, msg) || nc3_read_f64(nc, yname, y, msg) ||
+        nc3_read_f64(nc, uname, u, msg) || nc3_read_f64(nc, vname, v, msg))
+        return fail(MR_ERR_FORMAT, msg);
+    return mr_current_from_arrays((int64_t)x.size(), (int64_t)y.size(), x.data(), y.data(),
+                                  (int64_t)u.size(), u.data(), (int64_t)v.size(), v.data(), out);
+}
+
+int mr_bathy_constant(float h, mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    mr_field *f = new mr_field;
+    f->kind = MR_BATHY_CONSTANT;
+    f->h = h;
+    *out = f;
+    return MR_OK;
+}
+
+int mr_bathy_slope(float h0, float x0, float y0, float dhdx, float dhdy, mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    mr_field *f = new mr_field;
+    f->kind = MR_BATHY_SLOPE;
+    f->h0 = h0; f->x0 = x0; f->y0 = y0; f->dhdx = dhdx; f->dhdy = dhdy;
+    *out = f;
+    return MR_OK;
+}
+
+int mr_bathy_array(int64_t n, const float *a, mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (n < 0 || (n && !a)) return fail(MR_ERR_INVALID_ARGUMENT, "invalid array arguments");
+    mr_field *f = new mr_field;
+    f->kind = MR_BATHY_ARRAY;
+    f->an = n;
+    f->arr.assign(a, a + n * n);
+    *out = f;
+    return MR_OK;
+}
+
+int mr_current_constant(double u, double v, mr_field **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    mr_field *f = new mr_field;
+    f->kind = MR_CURRENT_CONSTANT;
+    f->u = u; f->v = v;
+    *out = f;
+    return MR_OK;
+}
+
+int mr_field_free(mr_field *f)
+{
+    delete f;
+    return MR_OK;
+}
+
+int mr_field_kind(const mr_field *f, int32_t *kind)
+{
+    if (!f || !kind) return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
+    *kind = f->kind;
+    return MR_OK;
+}
+
+int mr_field_grid_shape(const mr_field *f, int64_t *nx, int64_t *ny, int64_t *len_a,
+                        int64_t *len_b)
+{
+    if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
+    if (nx) *nx = f->nx;
+    if (ny) *ny = f->ny;
+    if (len_a) *len_a = (int64_t)f->a.size();
+    if (len_b) *len_b = (int64_t)f->b.size();
+    return MR_OK;
+}
+
+int mr_field_copy_grid(const mr_field *f, double *x, double *y, double *a, double *b)
+{
+    if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
+    if (f->kind != MR_BATHY_GRID && f->kind != MR_CURRENT_GRID)
+        return fail(MR_ERR_INVALID_ARGUMENT, "not a gridded field");
+    for (int64_t i = 0; i < f->nx; i++)
+        if (x) x[i] = f->kind == MR_BATHY_GRID ? (double)f->xf[(size_t)i] : f->xd[(size_t)i];
+    for (int64_t i = 0; i < f->ny; i++)
+        if (y) y[i] = f->kind == MR_BATHY_GRID ? (double)f->yf[(size_t)i] : f->yd[(size_t)i];
+    if (a) std::memcpy(a, f->a.data(), f->a.size() * sizeof(double));
+    if (b) std::memcpy(b, f->b.data(), f->b.size() * sizeof(double));
+    return MR_OK;
+}
+
+int mr_field_upload(const mr_field *f, int32_t device)
+{
+    if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
+    DeviceGuard g(device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    void *v;
+    return field_view(f, &v);
+}
+
+int mr_num_steps(double t0, double t_end, double dt, int64_t *n)
+{
+    if (!n) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
+    // Rk4::integrate: ((x_end - x) / step_size).ceil().to_usize().unwrap();
+    // num-traits accepts -1 < q < 2^64 and truncates
+    const double q = std::ceil((t_end - t0) / dt);
+    if (!(q > -1.0 && q < 18446744073709551616.0))
+        return fail(MR_ERR_STEP_COUNT, "step count ceil((t_end - t0)/dt) is not a valid usize");
+    if (q >= 2147483647.0) return fail(MR_ERR_STEP_COUNT, "step count too large");
+    *n = (int64_t)q;
+    return MR_OK;
+}
+
+int mr_trace_many(const mr_field *bathy, const mr_field *current, int64_t n_rays,
+                  const double *x0, const double *y0, const double *kx0, const double *ky0,
+                  double t0, double t_end, double dt, const mr_options *opt,
+                  double *t_out, double *x_out, double *y_out, double *kx_out,
+                  double *ky_out, int32_t *len_out, double *end_out)
+{
+    int rc = check_pair(bathy, current);
+    if (rc) return rc;
+    mr_options o;
+    if ((rc = resolve_options(opt, o))) return rc;
+    if (n_rays < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n_rays > 0 && (!x0 || !y0 || !kx0 || !ky0))
+        return fail(MR_ERR_INVALID_ARGUMENT, "null initial conditions");
+    int64_t n;
+    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+
+    if (o.memory == MR_MEM_DEVICE) {
+        return trace_device(bathy, current, n_rays, x0, y0, kx0, ky0, t0, n, dt, o,
+                            (hipStream_t)o.stream, t_out, x_out, y_out, kx_out, ky_out,
+                            len_out, end_out);
+    }
+    // host memory: stage through device buffers, synchronous
+    DevTables tmp;
+    const size_t nr = (size_t)n_rays, rows = (size_t)n + 1;
+    double *d_ic = nullptr, *d_t = nullptr, *d_end = nullptr;
+    double *d_out[4] = {nullptr, nullptr, nullptr, nullptr};
+    int32_t *d_len = nullptr;
+    double *h_out[4] = {x_out, y_out, kx_out, ky_out};
+    if ((rc = dev_alloc(tmp, 4 * nr, &d_ic))) return rc;
+    const double *h_ic[4] = {x0, y0, kx0, ky0};
+    for (int q = 0; q < 4; q++)
+        if (nr) MR_HIP_TRY(hipMemcpy(d_ic + q * nr, h_ic[q], nr * sizeof(double),
+                                     hipMemcpyHostToDevice));
+    if (t_out && (rc = dev_alloc(tmp, rows, &d_t))) return rc;
+    for (int q = 0; q < 4; q++)
+        if (h_out[q] && (rc = dev_alloc(tmp, rows * nr, &d_out[q]))) return rc;
+    if (len_out && (rc = dev_alloc(tmp, nr, &d_len))) return rc;
+    if (end_out && (rc = dev_alloc(tmp, 4 * nr, &d_end))) return rc;
+    rc = trace_device(bathy, current, n_rays, d_ic, d_ic + nr, d_ic + 2 * nr, d_ic + 3 * nr,
+                      t0, n, dt, o, 0, d_t, d_out[0], d_out[1], d_out[2], d_out[3], d_len,
+                      d_end);
+    if (rc) return rc;
+    MR_HIP_TRY(hipDeviceSynchronize());
+    if (t_out) MR_HIP_TRY(hipMemcpy(t_out, d_t, rows * sizeof(double), hipMemcpyDeviceToHost));
+    for (int q = 0; q < 4; q++)
+        if (h_out[q] && nr)
+            MR_HIP_TRY(hipMemcpy(h_out[q], d_out[q], rows * nr * sizeof(double),
+                                 hipMemcpyDeviceToHost));
+    if (len_out && nr)
+        MR_HIP_TRY(hipMemcpy(len_out, d_len, nr * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (end_out && nr)
+        MR_HIP_TRY(hipMemcpy(end_out, d_end, 4 * nr * sizeof(double), hipMemcpyDeviceToHost));
+    return MR_OK;
+}
+
+int mr_single_ray(const mr_field *bathy, const mr_field *current, double x0, double y0,
+                  double kx0, double ky0, double t0, double t_end, double dt,
+                  const mr_options *opt, double *rows_out, int64_t *n_rows)
+{
+    if (!rows_out || !n_rows) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
+    mr_options o;
+    int rc = resolve_options(opt, o);
+    if (rc) return rc;
+    if (o.memory != MR_MEM_HOST)
+        return fail(MR_ERR_INVALID_ARGUMENT, "mr_single_ray takes host memory");
+    int64_t n;
+    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
+    const size_t rows = (size_t)n + 1;
+    std::vector<double> t(rows), x(rows), y(rows), kx(rows), ky(rows);
+    int32_t len = 0;
+    rc = mr_trace_many(bathy, current, 1, &x0, &y0, &kx0, &ky0, t0, t_end, dt, &o, t.data(),
+                       x.data(), y.data(), kx.data(), ky.data(), &len, nullptr);
+    if (rc) return rc;
+    for (int32_t r = 0; r < len; r++) {
+        rows_out[5 * r + 0] = t[(size_t)r];
+        rows_out[5 * r + 1] = x[(size_t)r];
+        rows_out[5 * r + 2] = y[(size_t)r];
+        rows_out[5 * r + 3] = kx[(size_t)r];
+        rows_out[5 * r + 4] = ky[(size_t)r];
+    }
+    *n_rows = len;
+    return MR_OK;
+}
+
+int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const double *s,
+                double *ds, const mr_options *opt)
+{
+    int rc = check_pair(bathy, current);
+    if (rc) return rc;
+    mr_options o;
+    if ((rc = resolve_options(opt, o))) return rc;
+    if (n < 0 || (n && (!s || !ds))) return fail(MR_ERR_INVALID_ARGUMENT, "invalid arrays");
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    void *bv, *cv;
+    if ((rc = field_view(bathy, &bv)) || (rc = field_view(current, &cv))) return rc;
+    RhsArgs a;
+    a.B = bv;
+    a.C = cv;
+    a.n = n;
+    if (o.memory == MR_MEM_DEVICE) {
+        a.s = s; a.ds = ds;
+        MR_HIP_TRY(launch_rhs(a, bathy_kind_of(bathy), current_kind_of(current), o.math,
+                              (hipStream_t)o.stream));
+        return MR_OK;
+    }
+    DevTables tmp;
+    double *d_s, *d_ds;
+    if ((rc = dev_alloc(tmp, 4 * (size_t)n, &d_s)) || (rc = dev_alloc(tmp, 4 * (size_t)n, &d_ds)))
+        return rc;
+    if (n) MR_HIP_TRY(hipMemcpy(d_s, s, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    a.s = d_s; a.ds = d_ds;
+    MR_HIP_TRY(launch_rhs(a, bathy_kind_of(bathy), current_kind_of(current), o.math, 0));
+    MR_HIP_TRY(hipDeviceSynchronize());
+    if (n) MR_HIP_TRY(hipMemcpy(ds, d_ds, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return MR_OK;
+}
+
+}  // extern "C"

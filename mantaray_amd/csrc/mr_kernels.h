@@ -1,0 +1,57 @@
+// mr_kernels.h -- kernel argument blocks and launchers (host-visible).
+#pragma once
+
+#include "mr_device.h"
+
+namespace mr {
+
+// The field views live in device memory and are re-read (scalar loads) at
+// each right-hand-side evaluation; holding all of them in SGPRs across the
+// step loop would exceed the SGPR file and spill to VGPR lanes.
+struct TraceArgs {
+    const void *B;                     // BathyView in device memory
+    const void *C;                     // CurrentView in device memory
+    int64_t n_rays, n_steps;
+    double dt, half, sixth;
+    const double *x0, *y0, *kx0, *ky0;
+    double *xo, *yo, *kxo, *kyo;       // step-major, may be null
+    int32_t *len_out;                  // may be null
+    double *end_out;                   // 4 per ray, may be null
+    int solout;                        // 0 state, 1 next derivative
+};
+
+struct RhsArgs {
+    const void *B;
+    const void *C;
+    int64_t n;
+    const double *s;
+    double *ds;
+};
+
+struct BathyBuildArgs {
+    int64_t nx, ny;
+    const float2 *xp, *yp;
+    const double *raw;
+    int64_t oob_from;
+    double xspace, yspace;
+    BCell *cells;
+};
+
+struct CurrentBuildArgs {
+    int64_t nx, ny;
+    const float2 *xp, *yp;
+    const double *u, *v;
+    int64_t oob_from;
+    double xspace, yspace;
+    CCell *cells;
+    float2 *mtab;                      // null when the f32 axes are uniform
+};
+
+hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
+                        hipStream_t stream);
+hipError_t launch_rhs(const RhsArgs &a, int bk, int ck, int math, hipStream_t stream);
+hipError_t launch_fill_time(double t0, double dt, int64_t n, double *t, hipStream_t stream);
+hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
+hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
+
+}  // namespace mr

@@ -1,0 +1,131 @@
+"""Deterministic synthetic workloads of the many-ray integrator.
+
+The measured and parity-tested configurations, with every random draw taken
+from SplitMix64 (a counter-based generator: draw i of seed s is
+mix64(s + (i + 1) * 0x9E3779B97F4A7C15)), so the inputs are identical on any
+machine with no dependence on numpy's RNG.
+
+  linear_beach()   501 x 501 grid at 100 m, depth max(200 - 0.004 x, 0.1) m,
+                   zero current on the same grid; 128 rays from x = 1 km,
+                   y ~ U[5, 45] km, period 10 s, direction ~ U[-30, 30] deg;
+                   dt = 1 s for 1000 s.  (The reference's CPU-runnable case.)
+  seamount_jet()   1024 x 1024 grid at 200 m, depth 4000 m minus a Gaussian
+                   seamount (3800 m, sigma 15 km) at the centre; zonal jet
+                   U(y) = 1.5 sech^2((y - y_c)/20 km), V = 0; 65,536 rays at
+                   uniform interior points, period ~ U[6, 16] s, direction
+                   ~ U[0, 2 pi); dt = 1 s, 5000 steps; rays that leave the
+                   grid end early.
+The wavenumber magnitude solves the linear dispersion relation
+sigma^2 = g k tanh(k h) (g = 9.8, sigma = 2 pi / T) at the launch depth by
+Newton's method.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+G = 9.8
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
+    """Draws start .. start+count-1 of SplitMix64 seeded with `seed` (uint64)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """Sequential uniform draws in [0, 1) (53-bit) from SplitMix64."""
+
+    def __init__(self, seed: int):
+        self.seed = seed
+        self.pos = 0
+
+    def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+        z = splitmix64(self.seed, self.pos, n)
+        self.pos += n
+        u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+
+def wavenumber(sigma, h, iters: int = 60) -> np.ndarray:
+    """|k| with sigma^2 = g k tanh(k h), Newton from the larger of the deep-
+    and shallow-water limits (the dispersion function is concave in k)."""
+    sigma = np.asarray(sigma, dtype=np.float64)
+    h = np.asarray(h, dtype=np.float64)
+    k = np.maximum(sigma * sigma / G, sigma / np.sqrt(G * h))
+    for _ in range(iters):
+        th = np.tanh(k * h)
+        f = G * k * th - sigma * sigma
+        df = G * th + G * k * h * (1.0 - th * th)
+        k_new = k - f / df
+        if np.all(np.abs(k_new - k) <= 1e-15 * np.abs(k)):
+            k = k_new
+            break
+        k = k_new
+    return k
+
+
+def linear_beach(n_rays: int = 128, seed: int = 0) -> dict:
+    n = 501
+    x = (np.arange(n) * 100.0).astype(np.float32)
+    y = x.copy()
+    xd = x.astype(np.float64)
+    depth_row = np.maximum(200.0 - 0.004 * xd, 0.1)
+    depth = np.broadcast_to(depth_row[None, :], (n, n)).copy()      # depth[j, i]
+    zeros = np.zeros((n, n))
+    rs = Stream(seed)
+    y0 = rs.uniform(n_rays, 5000.0, 45000.0)
+    theta = np.deg2rad(rs.uniform(n_rays, -30.0, 30.0))
+    x0 = np.full(n_rays, 1000.0)
+    sigma = 2.0 * np.pi / 10.0
+    k = wavenumber(np.full(n_rays, sigma), np.maximum(200.0 - 0.004 * x0, 0.1))
+    return {
+        "name": "linear_beach",
+        "bathy": {"x": x, "y": y, "depth": depth},
+        "current": {"x": xd, "y": xd.copy(), "u": zeros, "v": zeros.copy()},
+        "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
+        "t0": 0.0, "t_end": 1000.0, "dt": 1.0,
+    }
+
+
+def seamount_jet(n_rays: int = 65536, seed: int = 1, n_steps: int = 5000) -> dict:
+    n = 1024
+    dx = 200.0
+    x = (np.arange(n) * dx).astype(np.float32)
+    xd = x.astype(np.float64)
+    xc = 0.5 * xd[-1]
+    r2 = (xd[None, :] - xc) ** 2 + (xd[:, None] - xc) ** 2
+    depth = 4000.0 - 3800.0 * np.exp(-r2 / (2.0 * 15000.0 ** 2))    # depth[j, i]
+    u_row = 1.5 / np.cosh((xd - xc) / 20000.0) ** 2                 # U(y)
+    u = np.broadcast_to(u_row[:, None], (n, n)).copy()              # u[j, i] = U(y_j)
+    v = np.zeros((n, n))
+    rs = Stream(seed)
+    lo, hi = dx, xd[-1] - dx
+    x0 = rs.uniform(n_rays, lo, hi)
+    y0 = rs.uniform(n_rays, lo, hi)
+    period = rs.uniform(n_rays, 6.0, 16.0)
+    theta = rs.uniform(n_rays, 0.0, 2.0 * np.pi)
+    h0 = 4000.0 - 3800.0 * np.exp(-((x0 - xc) ** 2 + (y0 - xc) ** 2) / (2.0 * 15000.0 ** 2))
+    k = wavenumber(2.0 * np.pi / period, h0)
+    return {
+        "name": "seamount_jet",
+        "bathy": {"x": x, "y": x.copy(), "depth": depth},
+        "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
+        "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
+        "t0": 0.0, "t_end": float(n_steps), "dt": 1.0,
+    }
+
+
+def make_fields(cfg: dict):
+    """Product field handles for a workload."""
+    from .fields import Bathymetry, Current
+    b = Bathymetry.from_arrays(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
+    c = Current.from_arrays(cfg["current"]["x"], cfg["current"]["y"], cfg["current"]["u"],
+                            cfg["current"]["v"])
+    return b, c

@@ -1,0 +1,128 @@
+"""Parity of the HIP path (libmantaray_hip.so, through its C ABI) with the
+CPU oracle, on the GPU.
+
+Tolerances:
+  - trajectories on the linear-beach workload: |dx|, |dy| <= 1e-6 m, the
+    bound the reference comparison is held to; identical row counts;
+  - right-hand side at arbitrary states: identical NaN pattern (every error
+    and termination case), finite values within 1e-12 relative (literal
+    transcendentals) / 1e-11 (fast dispersion form) of the column scale;
+  - the interpolated depth/current themselves are f32 values computed with
+    the reference's operations, so states whose derivatives depend only on
+    them (constant-depth rays) are compared exactly.
+"""
+import numpy as np
+import pytest
+
+from _helpers import compare_rhs, oracle_fields, random_grid_case, random_states
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mr():
+    import mantaray_amd
+    from mantaray_amd import _capi
+    _capi.lib()
+    return mantaray_amd
+
+
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_linear_beach_trajectories(mr, oracle_mod, math):
+    from mantaray_amd import workloads
+    cfg = workloads.linear_beach()
+    b, c = workloads.make_fields(cfg)
+    got = mr.trace_many(b, c, cfg["x0"], cfg["y0"], cfg["kx0"], cfg["ky0"], cfg["t0"],
+                        cfg["t_end"], cfg["dt"], math=math)
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    ref = oracle_mod.trace_many(ob, oc, cfg["x0"], cfg["y0"], cfg["kx0"], cfg["ky0"],
+                                cfg["t0"], cfg["t_end"], cfg["dt"])
+    assert np.array_equal(got["len"], ref["len"])
+    assert np.array_equal(got["t"], ref["t"])
+    for k in ("x", "y"):
+        assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k]))
+        d = np.nanmax(np.abs(got[k] - ref[k]))
+        print(f"{math} max |d{k}| = {d:.3e} m")
+        assert d <= 1e-6
+    for k in ("kx", "ky"):
+        d = np.nanmax(np.abs(got[k] - ref[k]) / np.abs(ref[k]).max())
+        assert d <= 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_rhs_random_states(mr, oracle_mod, seed, math):
+    cfg = random_grid_case(seed)
+    b, c = mr.Bathymetry.from_arrays(**cfg["bathy"]), mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 4096)
+    ds = mr.rhs(b, c, s, math=math)
+    ref, st = oracle_mod.system_many(ob, oc, s)
+    assert np.array_equal(np.isnan(ds), np.isnan(ref))
+    scale = np.nanmax(np.abs(ref), axis=0)
+    err = np.nanmax(np.abs(ds - ref) / scale)
+    print(f"seed {seed} {math}: max scaled error {err:.3e}, {int((st != 0).sum())} error rows")
+    assert err <= (1e-12 if math == "literal" else 1e-11)
+
+
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_reference_one_step_values(mr, math):
+    """wave_ray_path.rs test_axis (:405-416) and test_shallow (:521-548):
+    one RK4 step from (0, 0) with t0 = 0, tf = 1, dt = 1."""
+    c = mr.Current.constant(0.0, 0.0)
+    g2 = np.sqrt(9.8) / 2.0
+    for h, cases in (
+        (1000.0, [(0.0, 1.0, 0.0, g2), (1.0, 0.0, g2, 0.0), (0.0, -1.0, 0.0, -g2),
+                  (-1.0, 0.0, -g2, 0.0)]),
+        (0.1, [(0.0, 1.0, 6.031543168844801e-17, 0.9850257515953494),
+               (1.0, 0.0, 0.9850257515953494, 0.0),
+               (0.0, -1.0, 6.031543168844801e-17, -0.9850257515953494),
+               (-1.0, 0.0, -0.9850257515953494, 1.2063086337689602e-16)]),
+    ):
+        b = mr.Bathymetry.constant(h)
+        for kx, ky, xf, yf in cases:
+            r = mr.trace_many(b, c, [0.0], [0.0], [kx], [ky], 0.0, 1.0, 1.0, math=math)
+            x, y = r["x"][-1, 0], r["y"][-1, 0]
+            # the reference asserts |x - xf| < f64::EPSILON
+            assert abs(x - xf) < 4 * np.finfo(float).eps and abs(y - yf) < 4 * np.finfo(float).eps
+
+
+def test_reference_termination_cases(mr):
+    """wave_ray_path.rs test_solout (:601-626), test_zero_k (:362-381),
+    test_zero_h (:383-401), out_of_range_give_nan (:552-570)."""
+    c = mr.Current.constant(0.0, 0.0)
+    a = mr.Bathymetry.array(np.full((2, 2), 1000.0))
+    r = mr.trace_many(a, c, [0.0], [0.0], [0.0], [1.0], 0.0, 10.0, 1.0)
+    assert r["len"][0] == 3 and np.isnan(r["end"][0]).all()
+    r = mr.trace_many(mr.Bathymetry.constant(1000.0), c, [0.0], [0.0], [0.0], [0.0], 0.0, 10.0, 1.0)
+    assert np.isnan(r["end"][0]).all()
+    r = mr.trace_many(mr.Bathymetry.constant(0.0), c, [0.0], [0.0], [1.0], [1.0], 0.0, 10.0, 1.0)
+    assert np.isnan(r["end"][0]).all()
+    r = mr.trace_many(a, c, [0.0], [0.0], [0.0], [1.0], 0.0, 10.0, 1.0)
+    assert np.isnan(r["end"][0][:2]).all()
+
+
+def test_core_api_netcdf(mr, tmp_path):
+    """python/tests/test_core.py:40-78 on the HIP path: 3x3 NetCDF-3 files."""
+    from scipy.io import netcdf_file
+    with netcdf_file(tmp_path / "island.nc", "w") as f:
+        f.createDimension("x", 3); f.createDimension("y", 3)
+        f.createVariable("x", "f8", ("x",))[:] = [-1e4, 0, 1e4]
+        f.createVariable("y", "f8", ("y",))[:] = [-1e4, 0, 1e4]
+        f.createVariable("depth", "f8", ("x", "y"))[:] = 10_000 * np.ones((3, 3))
+    with netcdf_file(tmp_path / "current.nc", "w") as f:
+        f.createDimension("x", 3); f.createDimension("y", 3)
+        f.createVariable("x", "f8", ("x",))[:] = [-1e8, 0, 1e8]
+        f.createVariable("y", "f8", ("y",))[:] = [-1e8, 0, 1e8]
+        f.createVariable("u", "f8", ("x", "y"))[:] = 0.01 * np.ones((3, 3))
+        f.createVariable("v", "f8", ("x", "y"))[:] = 0.01 * np.ones((3, 3))
+    ds = mr.single_ray(-1000, 0, 0.01, 0, 10, 2, tmp_path / "island.nc", tmp_path / "current.nc")
+    assert ds.sizes["time_step"] == 6
+    assert (ds.kx == 0.01).all()
+    assert (ds.ky == 0.0).all()
+    ds = mr.ray_tracing(3 * [-1000], 3 * [0], 3 * [0.01], 3 * [0], 10, 2,
+                        str(tmp_path / "island.nc"), str(tmp_path / "current.nc"))
+    assert ds.sizes["time_step"] == 6
+    assert ds.sizes["ray"] == 3
+    assert (ds.kx == 0.01).all()
+    assert (ds.ky == 0.0).all()
