@@ -43,6 +43,8 @@ bool is_current(int32_t k) { return k == MR_CURRENT_CONSTANT || k == MR_CURRENT_
 struct DevTables {
     std::vector<void *> allocs;
     void *view = nullptr;          // BathyView or CurrentView in device memory
+    BathyPtrs bp{};                // device pointers of a bathymetry
+    CurrentPtrs cp{};              // device pointers of a current
     ~DevTables()
     {
         for (void *p : allocs) (void)hipFree(p);
@@ -152,6 +154,112 @@ int validate_axes_f64(const std::vector<double> &x, const char *name)
     return MR_OK;
 }
 
+// grids the kernels index with 32-bit cell byte offsets
+int validate_grid(int64_t nx, int64_t ny, int64_t rec_bytes)
+{
+    if (nx >= 2 && ny >= 2 &&
+        ((nx - 1 + 7) / 8) * ((ny - 1 + 7) / 8) * 64 * rec_bytes >= (int64_t(1) << 32))
+        return fail(MR_ERR_UNSUPPORTED, "grid too large for 32-bit cell offsets");
+    return MR_OK;
+}
+
+// ---- cell thresholds -------------------------------------------------------
+// The reference's cell along an axis is a monotone step function of the
+// coordinate: the fractional index RN(RN(p - a0) / sp) is non-decreasing in
+// p (each rounding is monotone), and the cell is its floor, clamped to n-2 on
+// the last node.  T[i] = min{p : index(p) >= i} for i < n-1 and T[n-1] =
+// min{p : index(p) > n-1} make cell i exactly [T[i], T[i+1]) and the valid
+// range [T[0], T[n-1]).  Each T is found by bisection over the totally
+// ordered floating-point values, evaluating the index with the reference's
+// own operations (cartesian_netcdf3.rs:285, cartesian_current.rs:244).
+
+uint32_t key32(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+float unkey32(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+uint64_t key64(double f)
+{
+    uint64_t b;
+    std::memcpy(&b, &f, 8);
+    return (b >> 63) ? ~b : (b | (uint64_t(1) << 63));
+}
+double unkey64(uint64_t k)
+{
+    uint64_t b = (k >> 63) ? (k & ~(uint64_t(1) << 63)) : ~k;
+    double f;
+    std::memcpy(&f, &b, 8);
+    return f;
+}
+
+// f32 axis (bathymetry): thresholds T[0..n-1]
+std::vector<float> thresholds32(const std::vector<float> &a)
+{
+    const size_t n = a.size();
+    const float a0 = a[0], sp = std::fabs(a[1] - a[0]);
+    auto index = [&](float p) { return (p - a0) / sp; };        // :285
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; i++) {
+        const float lim = (float)i;
+        const bool upper = i == n - 1;                         // index > n-1 errs
+        auto pred = [&](float p) { return upper ? index(p) > lim : index(p) >= lim; };
+        uint32_t lo = key32(-INFINITY), hi = key32(INFINITY);  // pred(lo) false, pred(hi) true
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (pred(unkey32(mid))) hi = mid; else lo = mid;
+        }
+        t[i] = unkey32(hi);
+    }
+    return t;
+}
+
+// f64 axis (current): thresholds T[0..n-1]
+std::vector<double> thresholds64(const std::vector<double> &a)
+{
+    const size_t n = a.size();
+    const double a0 = a[0], sp = std::fabs(a[1] - a[0]);
+    auto index = [&](double p) { return (p - a0) / sp; };       // :244
+    std::vector<double> t(n);
+    for (size_t i = 0; i < n; i++) {
+        const double lim = (double)i;
+        const bool upper = i == n - 1;
+        auto pred = [&](double p) { return upper ? index(p) > lim : index(p) >= lim; };
+        uint64_t lo = key64(-INFINITY), hi = key64(INFINITY);
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (pred(unkey64(mid))) hi = mid; else lo = mid;
+        }
+        t[i] = unkey64(hi);
+    }
+    return t;
+}
+
+// per-cell axis data of the bathymetry: (a[i], a[i+1], T[i], T[i+1])
+std::vector<float4> axis_cells32(const std::vector<float> &a)
+{
+    const std::vector<float> t = thresholds32(a);
+    std::vector<float4> c(a.size() - 1);
+    for (size_t i = 0; i + 1 < a.size(); i++) c[i] = make_float4(a[i], a[i + 1], t[i], t[i + 1]);
+    return c;
+}
+
+// per-cell thresholds of the current: (T[i], T[i+1])
+std::vector<double2> axis_cells64(const std::vector<double> &a)
+{
+    const std::vector<double> t = thresholds64(a);
+    std::vector<double2> c(a.size() - 1);
+    for (size_t i = 0; i + 1 < a.size(); i++) c[i] = make_double2(t[i], t[i + 1]);
+    return c;
+}
+
 std::vector<float2> axis_pairs(const std::vector<float> &a)
 {
     std::vector<float2> p(a.size() - 1);
@@ -169,32 +277,41 @@ int build_bathy(const mr_field &f, DevTables &t)
     } else if (f.kind == MR_BATHY_SLOPE) {
         v.h0 = f.h0; v.x0 = f.x0; v.y0 = f.y0; v.dhdx = f.dhdx; v.dhdy = f.dhdy;
     } else if (f.kind == MR_BATHY_ARRAY) {
+        if (f.an > 46340) return fail(MR_ERR_UNSUPPORTED, "ArrayDepth too large");
         float *d = nullptr;
         int rc = dev_upload(t, f.arr.data(), f.arr.size(), &d);
         if (rc) return rc;
-        v.arr = d;
-        v.an = f.an;
+        t.bp.arr = d;
+        v.an = (int)f.an;
     } else {
+        int rc = validate_grid(f.nx, f.ny, 32);
+        if (!rc) rc = validate_axes_f32(f.xf, "x");
+        if (!rc) rc = validate_axes_f32(f.yf, "y");
+        if (rc) return rc;
         v.nx = (int)f.nx;
         v.ny = (int)f.ny;
+        v.ncx = (int)(f.nx - 1);
         v.ax0 = f.xf[0];
         v.ay0 = f.yf[0];
         v.nxm1 = (float)(f.nx - 1);                 // (array.len() - 1) as f32
         v.nym1 = (float)(f.ny - 1);
-        v.nxm2 = (float)(f.nx - 2);
-        v.nym2 = (float)(f.ny - 2);
         v.inv_sx = 1.0 / (double)std::fabs(f.xf[1] - f.xf[0]);   // spacing in f32
         v.inv_sy = 1.0 / (double)std::fabs(f.yf[1] - f.yf[0]);
         std::vector<float2> xp = axis_pairs(f.xf), yp = axis_pairs(f.yf);
+        std::vector<float4> xa = axis_cells32(f.xf), ya = axis_cells32(f.yf);
         float2 *dxp, *dyp;
+        float4 *dxa, *dya;
         double *draw;
-        int rc = dev_upload(t, xp.data(), xp.size(), &dxp);
+        rc = dev_upload(t, xp.data(), xp.size(), &dxp);
         if (!rc) rc = dev_upload(t, yp.data(), yp.size(), &dyp);
+        if (!rc) rc = dev_upload(t, xa.data(), xa.size(), &dxa);
+        if (!rc) rc = dev_upload(t, ya.data(), ya.size(), &dya);
         if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &draw);
         if (rc) return rc;
-        const int64_t ncell = (f.nx - 1) * (f.ny - 1);
+        const int ntx = (int)((f.nx - 1 + 7) / 8), nty = (int)((f.ny - 1 + 7) / 8);
+        const int64_t ntile_cells = (int64_t)ntx * nty * 64;     // 8 x 8-cell tiles
         BCell *cells;
-        rc = dev_alloc(t, (size_t)ncell, &cells);
+        rc = dev_alloc(t, (size_t)ntile_cells, &cells);
         if (rc) return rc;
         BathyBuildArgs ba;
         ba.nx = f.nx; ba.ny = f.ny; ba.xp = dxp; ba.yp = dyp; ba.raw = draw;
@@ -202,8 +319,11 @@ int build_bathy(const mr_field &f, DevTables &t)
         ba.xspace = (double)f.xf[1] - (double)f.xf[0];   // cartesian_netcdf3.rs:119-120
         ba.yspace = (double)f.yf[1] - (double)f.yf[0];
         ba.cells = cells;
+        ba.ntx = ntx;
         MR_HIP_TRY(launch_build_bathy(ba, 0));
-        v.xp = dxp; v.yp = dyp; v.cells = cells; v.oob_from = ba.oob_from; v.raw = draw;
+        v.oob_from = (uint32_t)ba.oob_from;
+        v.ntx = ntx;
+        t.bp.xa = dxa; t.bp.ya = dya; t.bp.cells = cells; t.bp.raw = draw;
     }
     BathyView *dv;
     int rc = dev_upload(t, &v, 1, &dv);
@@ -221,8 +341,13 @@ int build_current(const mr_field &f, DevTables &t)
     if (f.kind == MR_CURRENT_CONSTANT) {
         v.u = f.u; v.v = f.v;
     } else {
+        int rc = validate_grid(f.nx, f.ny, 64);
+        if (!rc) rc = validate_axes_f64(f.xd, "x");
+        if (!rc) rc = validate_axes_f64(f.yd, "y");
+        if (rc) return rc;
         v.nx = (int)f.nx;
         v.ny = (int)f.ny;
+        v.ncx = (int)(f.nx - 1);
         v.cx0 = f.xd[0];
         v.cy0 = f.yd[0];
         v.sx = std::fabs(f.xd[1] - f.xd[0]);      // cartesian_current.rs:242
@@ -231,8 +356,6 @@ int build_current(const mr_field &f, DevTables &t)
         v.inv_sy = 1.0 / v.sy;
         v.nxm1 = (double)(f.nx - 1);
         v.nym1 = (double)(f.ny - 1);
-        v.nxm2 = (double)(f.nx - 2);
-        v.nym2 = (double)(f.ny - 2);
         std::vector<float> x32(f.xd.size()), y32(f.yd.size());
         for (size_t i = 0; i < x32.size(); i++) x32[i] = (float)f.xd[i];   // :375-378
         for (size_t i = 0; i < y32.size(); i++) y32[i] = (float)f.yd[i];
@@ -252,18 +375,23 @@ int build_current(const mr_field &f, DevTables &t)
             }
         }
         std::vector<float2> xp = axis_pairs(x32), yp = axis_pairs(y32);
+        std::vector<double2> xt = axis_cells64(f.xd), yt = axis_cells64(f.yd);
         float2 *dxp, *dyp;
+        double2 *dxt, *dyt;
         double *du, *dv;
-        int rc = dev_upload(t, xp.data(), xp.size(), &dxp);
+        rc = dev_upload(t, xp.data(), xp.size(), &dxp);
         if (!rc) rc = dev_upload(t, yp.data(), yp.size(), &dyp);
+        if (!rc) rc = dev_upload(t, xt.data(), xt.size(), &dxt);
+        if (!rc) rc = dev_upload(t, yt.data(), yt.size(), &dyt);
         if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &du);
         if (!rc) rc = dev_upload(t, f.b.data(), f.b.size(), &dv);
         if (rc) return rc;
-        const int64_t ncell = (f.nx - 1) * (f.ny - 1);
+        const int ntx = (int)((f.nx - 1 + 7) / 8), nty = (int)((f.ny - 1 + 7) / 8);
+        const int64_t ntile_cells = (int64_t)ntx * nty * 64;     // 8 x 8-cell tiles
         CCell *cells;
         float2 *mtab = nullptr;
-        rc = dev_alloc(t, (size_t)ncell, &cells);
-        if (!rc && !uniform) rc = dev_alloc(t, (size_t)ncell, &mtab);
+        rc = dev_alloc(t, (size_t)ntile_cells, &cells);
+        if (!rc && !uniform) rc = dev_alloc(t, (size_t)ntile_cells, &mtab);
         if (rc) return rc;
         CurrentBuildArgs ca;
         ca.nx = f.nx; ca.ny = f.ny; ca.xp = dxp; ca.yp = dyp; ca.u = du; ca.v = dv;
@@ -273,9 +401,13 @@ int build_current(const mr_field &f, DevTables &t)
         ca.yspace = f.yd[1] - f.yd[0];
         ca.cells = cells;
         ca.mtab = mtab;
+        ca.ntx = ntx;
         MR_HIP_TRY(launch_build_current(ca, 0));
-        v.xp = dxp; v.yp = dyp; v.cells = cells; v.oob_from = ca.oob_from; v.mtab = mtab;
-        v.raw_u = du; v.raw_v = dv;
+        v.oob_from = (uint32_t)ca.oob_from;
+        v.ntx = ntx;
+        t.cp.xp = dxp; t.cp.yp = dyp; t.cp.xt = dxt; t.cp.yt = dyt;
+        t.cp.cells = cells; t.cp.mtab = mtab;
+        t.cp.raw_u = du; t.cp.raw_v = dv;
     }
     CurrentView *dview;
     int rc = dev_upload(t, &v, 1, &dview);
@@ -286,7 +418,7 @@ int build_current(const mr_field &f, DevTables &t)
 }
 
 // the device view of a field on the current device, built on first use
-int field_view(const mr_field *cf, void **view)
+int field_tables(const mr_field *cf, const DevTables **out)
 {
     mr_field *f = const_cast<mr_field *>(cf);
     int dev = 0;
@@ -294,13 +426,13 @@ int field_view(const mr_field *cf, void **view)
     std::lock_guard<std::mutex> lk(f->mu);
     auto it = f->dev.find(dev);
     if (it != f->dev.end()) {
-        *view = it->second->view;
+        *out = it->second.get();
         return MR_OK;
     }
     std::unique_ptr<DevTables> t(new DevTables);
     int rc = is_bathy(f->kind) ? build_bathy(*f, *t) : build_current(*f, *t);
     if (rc) return rc;
-    *view = t->view;
+    *out = t.get();
     f->dev[dev] = std::move(t);
     return MR_OK;
 }
@@ -350,16 +482,20 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                  double *t_out, double *x_out, double *y_out, double *kx_out, double *ky_out,
                  int32_t *len_out, double *end_out)
 {
-    void *bv, *cv;
-    int rc = field_view(bathy, &bv);
-    if (!rc) rc = field_view(current, &cv);
+    const DevTables *bt, *ct;
+    int rc = field_tables(bathy, &bt);
+    if (!rc) rc = field_tables(current, &ct);
     if (rc) return rc;
-    if (t_out) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
+    if (t_out && n_rays == 0) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
     TraceArgs a;
-    a.B = bv;
-    a.C = cv;
+    a.B = bt->view;
+    a.C = ct->view;
+    a.bp = bt->bp;
+    a.cp = ct->cp;
     a.n_rays = n_rays;
     a.n_steps = n;
+    a.t0 = t0;
+    a.t_out = t_out;
     a.dt = dt;
     a.half = dt / 2.0;                                 // Rk4::new half_step
     a.sixth = dt / 6.0;                                // Rk4::step (step_size / 6)
@@ -415,9 +551,6 @@ int mr_bathy_from_arrays(int64_t nx, int64_t ny, const float *x, const float *y,
     f->xf.assign(x, x + nx);
     f->yf.assign(y, y + ny);
     f->a.assign(depth, depth + depth_len);
-    int rc = validate_axes_f32(f->xf, "x");
-    if (!rc) rc = validate_axes_f32(f->yf, "y");
-    if (rc) return rc;
     *out = f.release();
     return MR_OK;
 }
@@ -438,9 +571,6 @@ int mr_current_from_arrays(int64_t nx, int64_t ny, const double *x, const double
     f->yd.assign(y, y + ny);
     f->a.assign(u, u + u_len);
     f->b.assign(v, v + v_len);
-    int rc = validate_axes_f64(f->xd, "x");
-    if (!rc) rc = validate_axes_f64(f->yd, "y");
-    if (rc) return rc;
     *out = f.release();
     return MR_OK;
 }
@@ -570,8 +700,8 @@ int mr_field_upload(const mr_field *f, int32_t device)
     if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
     DeviceGuard g(device);
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
-    void *v;
-    return field_view(f, &v);
+    const DevTables *t;
+    return field_tables(f, &t);
 }
 
 int mr_num_steps(double t0, double t_end, double dt, int64_t *n)
@@ -683,11 +813,13 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     if (n < 0 || (n && (!s || !ds))) return fail(MR_ERR_INVALID_ARGUMENT, "invalid arrays");
     DeviceGuard g(o.device);
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
-    void *bv, *cv;
-    if ((rc = field_view(bathy, &bv)) || (rc = field_view(current, &cv))) return rc;
+    const DevTables *bt, *ct;
+    if ((rc = field_tables(bathy, &bt)) || (rc = field_tables(current, &ct))) return rc;
     RhsArgs a;
-    a.B = bv;
-    a.C = cv;
+    a.B = bt->view;
+    a.C = ct->view;
+    a.bp = bt->bp;
+    a.cp = ct->cp;
     a.n = n;
     if (o.memory == MR_MEM_DEVICE) {
         a.s = s; a.ds = ds;

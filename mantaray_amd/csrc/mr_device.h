@@ -3,29 +3,36 @@
 //
 // Citations are to files of mines-oceanography/mantaray 0.1.0.
 //
-// Data layout in HBM (built once per field and device by build_bathy_kernel and
-// build_current_kernel in mr_kernels.hip):
+// Data layout in HBM (built once per field and device by build_bathy_kernel
+// and build_current_kernel in mr_kernels.hip):
 //   gridded bathymetry: one 32-byte record per cell (i, j), i < nx-1,
-//     j < ny-1, row-major with stride nx-1, holding the f32 bilinear
+//     j < ny-1, in 8 x 8-cell tiles (tiled_cell), holding the f32 bilinear
 //     coefficients of the cell, the f32 cell gradient and the cell's
-//     change-of-basis entries; plus the f32 axes as (a[i], a[i+1]) pairs
-//     and the raw f64 depth (read only when the target is exactly a node).
+//     change-of-basis entries; the f32 axes as (a[i], a[i+1]) pairs; the raw
+//     f64 depth (read only when the target is exactly a grid node).
 //   gridded current: one 64-byte record per cell with the f32 bilinear
 //     coefficients of u and v and the four f64 cell gradients; pairs of the
-//     f32 casts of the f64 axes; the change of basis as two constants when the f32 axes
-//     are uniform, else a per-cell float2 table; the raw f64 u and v.
-// A query loads one record per field (dwordx4 loads) and the two axis pairs
-// around the cell instead of 12 scattered corner values, and performs no
-// division: everything cell-constant is precomputed with the reference's own
+//     f32 casts of the f64 axes; the change of basis as two constants when
+//     the f32 axes are uniform, else a per-cell float2 table; the raw f64 u
+//     and v.
+// A query loads one record per field and the two axis pairs around the cell
+// instead of 12 scattered corner values, and performs no division:
+// everything cell-constant is precomputed with the reference's own
 // operations, so the interpolated values are bitwise those of the reference.
-// Cells on which the reference errs are recognised without extra loads (see
-// the interpolation notes below).
+//
+// Register layout in the kernels: the numeric parameters of a field (a
+// *View) are held per lane in VGPRs; its device pointers (a *Ptrs) are kernel
+// arguments in SGPRs, so record loads are global loads with a scalar base and
+// a 32-bit lane offset.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace mr {
+
+#define MR_NANF_INIT __builtin_nanf("")
+#define MR_NAN_INIT __builtin_nan("")
 
 constexpr double G = 9.8;                        // src/wave_ray_path.rs:23
 
@@ -49,36 +56,112 @@ struct BathyView {
     int kind;
     float h;                              // ConstantDepth
     float h0, x0, y0, dhdx, dhdy;         // ConstantSlope
-    const float *arr; int64_t an;         // ArrayDepth (an x an)
-    int nx, ny;                           // CartesianNetcdf3
-    float ax0, ay0, nxm1, nym1, nxm2, nym2;   // a[0]; (f32)(n-1); (f32)(n-2)
+    int an;                               // ArrayDepth (an x an)
+    int nx, ny, ncx;                      // CartesianNetcdf3; ncx = nx - 1
+    float ax0, ay0, nxm1, nym1;           // a[0]; (f32)(n-1)
     double inv_sx, inv_sy;                // RN64(1/(double)|a1-a0|)
-    const float2 *xp, *yp;                // axis pairs (a[i], a[i+1]), i < n-1
+    uint32_t oob_from;                    // first cell (row-major) with a corner past the data
+    int ntx;                              // 8 x 8 record tiles along x
+};
+
+struct BathyPtrs {
+    const float *arr;                     // ArrayDepth values
+    const float4 *xa, *ya;                // per axis cell i < n-1: (a[i], a[i+1], T[i], T[i+1])
     const BCell *cells;
-    int64_t oob_from;                     // first cell with a corner past the data
     const double *raw;                    // raw depth (exact-corner values)
 };
 
 struct CurrentView {
     int kind;
+    int nx, ny, ncx;                      // CartesianCurrent; ncx = nx - 1
     double u, v;                          // ConstantCurrent
-    int nx, ny;                           // CartesianCurrent
     double cx0, cy0, sx, sy, inv_sx, inv_sy;
-    double nxm1, nym1, nxm2, nym2;        // n-1, n-2
+    double nxm1, nym1;                    // n - 1
+    float m01, m10;                       // change of basis of uniform axes
+    uint32_t oob_from;                    // first cell (row-major) with a corner past u or v
+    int ntx;                              // 8 x 8 record tiles along x
+};
+
+struct CurrentPtrs {
     const float2 *xp, *yp;                // pairs of the (f32) casts of the f64 axes
+    const double2 *xt, *yt;               // per axis cell: (T[i], T[i+1]) in f64
     const CCell *cells;
-    int64_t oob_from;                     // first cell with a corner past u or v
-    const float2 *mtab;                   // per-cell (m01, m10) or null
-    float m01, m10;                       // uniform axes
+    const float2 *mtab;                   // per-cell (m01, m10), or null if uniform
     const double *raw_u, *raw_v;          // raw values (exact-corner values)
 };
 
-// Read-only device memory seen through the constant address space, so that
-// uniform loads from it are scalar (s_load) loads.
-#define MR_CONST __attribute__((address_space(4)))
+// Per-lane copy of the last cell record read, with the cell's index range.
+// Along each axis the cell the reference chooses is a monotone step function
+// of the coordinate (cartesian_netcdf3.rs:274-390, cartesian_current.rs:231-339
+// round, divide and floor monotonically), so cell i is exactly the interval
+// T[i] <= p < T[i+1] of a per-axis threshold table T (T[0] the first valid
+// coordinate, T[n-1] the first one past the last node).  The thresholds are
+// found on the host by bisection over the floating-point values with the
+// reference's own arithmetic.  A ray stays in one cell for many evaluations;
+// while it does, the query costs four comparisons per field and no index
+// arithmetic or load.  Thresholds start as NaN so that the first query
+// misses.
+struct BathySlot {
+    float4 r0 = {0.f, 0.f, 0.f, 0.f}, r1 = {0.f, 0.f, 0.f, 0.f};
+    float4 ax = {0.f, 0.f, MR_NANF_INIT, MR_NANF_INIT};   // (x_i, x_i+1, T_i, T_i+1)
+    float4 ay = {0.f, 0.f, MR_NANF_INIT, MR_NANF_INIT};
+    int i1 = 0, j1 = 0;
+};
+
+struct CurrentSlot {
+    float4 r0 = {0.f, 0.f, 0.f, 0.f}, r1 = {0.f, 0.f, 0.f, 0.f};
+    double2 g0 = {0.0, 0.0}, g1 = {0.0, 0.0};
+    float2 xp = {0.f, 0.f}, yp = {0.f, 0.f}, m = {0.f, 0.f};
+    double2 tx = {MR_NAN_INIT, MR_NAN_INIT}, ty = {MR_NAN_INIT, MR_NAN_INIT};
+    int i1 = 0, j1 = 0;
+};
+
+struct BathyCache {
+    BathySlot cur;
+};
+
+struct CurrentCache {
+    CurrentSlot cur;
+};
+
+// Loads from device (global) memory through the global address space, so
+// that the compiler emits global (not flat) loads.
+#define MR_GLOBAL __attribute__((address_space(1)))
+typedef float mr_f2 __attribute__((ext_vector_type(2)));
+typedef float mr_f4 __attribute__((ext_vector_type(4)));
+typedef double mr_d2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ T gld(const T *p, uint32_t i)
+{
+    return *(const MR_GLOBAL T *)((const MR_GLOBAL char *)p + i * (uint32_t)sizeof(T));
+}
+__device__ __forceinline__ float4 gld4(const void *p, uint32_t byte_off)
+{
+    const mr_f4 v = *(const MR_GLOBAL mr_f4 *)((const MR_GLOBAL char *)p + byte_off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double2 gldd2(const void *p, uint32_t byte_off)
+{
+    const mr_d2 v = *(const MR_GLOBAL mr_d2 *)((const MR_GLOBAL char *)p + byte_off);
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 gld2(const float2 *p, uint32_t i)
+{
+    const mr_f2 v = *(const MR_GLOBAL mr_f2 *)((const MR_GLOBAL char *)p + i * 8u);
+    return make_float2(v.x, v.y);
+}
 
 #define MR_NAN __builtin_nan("")
 #define MR_NANF __builtin_nanf("")
+
+// Records are stored in 8 x 8-cell tiles (tile-major, row-major inside a
+// tile), so that the neighbours a ray moves into lie on the same memory page
+// as the cell it leaves.  ntx = number of tiles along x.
+__device__ __forceinline__ uint32_t tiled_cell(int i, int j, int ntx)
+{
+    const uint32_t t = (uint32_t)(j >> 3) * (uint32_t)ntx + (uint32_t)(i >> 3);
+    return (t << 6) | ((uint32_t)(j & 7) << 3) | (uint32_t)(i & 7);
+}
 
 // ---------------------------------------------------------------------------
 // Interpolation from cell records.
@@ -98,6 +181,10 @@ struct CurrentView {
 //   - det == 0 (interpolator.rs:65-67): Err.  The record build marks such a
 //     cell with m01 = NaN; no other cell has a NaN m01 (det is finite and
 //     nonzero there, so dtx/det is finite or infinite).
+// A query reports Err by returning NaN for the depth (bathymetry) or by
+// clearing its ok flag (current): the right-hand side then yields all-NaN
+// derivatives, which is what the reference's Err produces
+// (wave_ray_path.rs:222-227).
 
 __device__ __forceinline__ float bilin_eval(float a00, float a10, float a01, float a11,
                                             float u, float w)
@@ -105,14 +192,24 @@ __device__ __forceinline__ float bilin_eval(float a00, float a10, float a01, flo
     return ((a00 + a10 * u) + a01 * w) + (a11 * u) * w;   // interpolator.rs:83
 }
 
-// index of the first corner (order A, B, C, D) equal to the target
-__device__ __forceinline__ int64_t corner_index(float px, float py, float xa, float xd,
-                                                float ya, float yb, int64_t nx, int i1,
-                                                int j1)
+// true when the target is exactly on one of the cell's corners: t - a is 0
+// only for t == a (subtraction of distinct finite floats is never zero)
+__device__ __forceinline__ bool on_corner(float dxa, float dxd, float dya, float dyb)
 {
-    const int64_t iA = nx * j1 + i1;
-    if (px == xa) return py == ya ? iA : iA + nx;          // A, B
-    return py == yb ? iA + nx + 1 : iA + 1;                // C, D
+    const float cx = __builtin_fminf(__builtin_fabsf(dxa), __builtin_fabsf(dxd));
+    const float cy = __builtin_fminf(__builtin_fabsf(dya), __builtin_fabsf(dyb));
+    return __builtin_fmaxf(cx, cy) == 0.0f;
+}
+
+// flat index (row stride nx) of the first corner, in the order A, B, C, D,
+// equal to the target
+__device__ __forceinline__ uint32_t corner_index(float dxa, float dya, float dyb, int nx,
+                                                 int i1, int j1)
+{
+    uint32_t ic = (uint32_t)nx * (uint32_t)j1 + (uint32_t)i1;
+    if (dxa == 0.0f) ic += dya == 0.0f ? 0u : (uint32_t)nx;
+    else ic += dyb == 0.0f ? (uint32_t)nx + 1u : 1u;
+    return ic;
 }
 
 // f32 fractional index RN32(t / sp) (cartesian_netcdf3.rs:285) as
@@ -128,133 +225,174 @@ __device__ __forceinline__ float index32(float t, double inv_sp)
 // f64 fractional index RN64(t / sp) (cartesian_current.rs:244), reduced to
 // what the cell choice needs: the bound checks and floor.  The product with
 // the rounded reciprocal is within 2^-50 (relative) of the quotient; only
-// when it lies that close to an integer is the exact division taken.
-// ok is cleared where the reference returns Err (a NaN index errs later, in
-// bilinear, because all four corners collapse onto (0, 0)); i1 is always a
-// valid cell index.
-__device__ __forceinline__ int index64(double p, double a0, double sp, double inv_sp,
-                                       double nm1, double nm2, bool &ok)
+// when it lies that close to an integer is the exact division taken.  ok is
+// cleared where the reference returns Err (a NaN index errs later, in
+// bilinear, because all four corners collapse onto (0, 0)).  The returned
+// floor is meaningful only when ok is set.
+__device__ __forceinline__ double index64(double p, double a0, double sp, double inv_sp,
+                                          double nm1, bool &ok)
 {
     const double t = p - a0;
     double q = t * inv_sp;
-    double fl = floor(q);
-    const double fr = q - fl;
-    const double tol = q * 0x1p-50;
-    if (fr < tol || fr > 1.0 - tol) {
+    double fl = __builtin_floor(q);
+    if (__builtin_fabs((q - fl) - 0.5) > 0.5 - q * 0x1p-50) {   // near an integer
         q = t / sp;
-        fl = floor(q);
+        fl = __builtin_floor(q);
     }
-    ok = ok && t >= 0.0 && q <= nm1;
-    return (int)__builtin_fmin(__builtin_fmax(fl, 0.0), nm2);
+    ok = ok & (t >= 0.0) & (q <= nm1);
+    return fl;
 }
 
 // ---------------------------------------------------------------------------
-// bathymetry: BathymetryData::depth_and_gradient (bathymetry/mod.rs:35-41)
+// bathymetry: BathymetryData::depth_and_gradient (bathymetry/mod.rs:35-41);
+// returns the depth widened to f64, NaN where the reference errs.
+
+__device__ __forceinline__ bool bathy_inside(const BathySlot &s, float px, float py)
+{
+    return (px >= s.ax.z) & (px < s.ax.w) & (py >= s.ay.z) & (py < s.ay.w);
+}
+
+__device__ __forceinline__ void bathy_load(const BathyView &B, const BathyPtrs &P, BathySlot &s,
+                                           int i1, int j1)
+{
+    const uint32_t cell = tiled_cell(i1, j1, B.ntx);
+    s.r0 = gld4(P.cells, cell * 32u);
+    s.r1 = gld4(P.cells, cell * 32u + 16u);
+    s.ax = gld4(P.xa, (uint32_t)i1 * 16u);                  // x[i1], x[i1+1], T
+    s.ay = gld4(P.ya, (uint32_t)j1 * 16u);
+    s.i1 = i1;
+    s.j1 = j1;
+}
 
 template <int BK>
-__device__ __forceinline__ bool bathy_query(const MR_CONST BathyView *B, double x, double y, double &h,
-                                            double &hx, double &hy)
+__device__ __forceinline__ double bathy_query(const BathyView &B, const BathyPtrs &P,
+                                              BathyCache &bc, double x, double y, double &hx,
+                                              double &hy)
 {
     const float px = (float)x, py = (float)y;             // wave_ray_path.rs:122
     if (BK == BK_CONSTANT) {                               // constant_depth.rs:34-40
-        bool nn = __builtin_isnan(px) || __builtin_isnan(py);
-        h = nn ? MR_NAN : (double)B->h;
-        hx = nn ? MR_NAN : 0.0;
-        hy = nn ? MR_NAN : 0.0;
-        return true;
+        const bool nn = __builtin_isnan(px) || __builtin_isnan(py);
+        hx = 0.0; hy = 0.0;
+        return nn ? MR_NAN : (double)B.h;
     } else if (BK == BK_SLOPE) {                           // constant_slope.rs:63-74
-        bool nn = __builtin_isnan(px) || __builtin_isnan(py);
-        float hv = (B->h0 + B->dhdx * (px - B->x0)) + B->dhdy * (py - B->y0);
-        h = nn ? MR_NAN : (double)hv;
-        hx = nn ? MR_NAN : (double)B->dhdx;
-        hy = nn ? MR_NAN : (double)B->dhdy;
-        return true;
+        const bool nn = __builtin_isnan(px) || __builtin_isnan(py);
+        const float hv = (B.h0 + B.dhdx * (px - B.x0)) + B.dhdy * (py - B.y0);
+        hx = (double)B.dhdx; hy = (double)B.dhdy;
+        return nn ? MR_NAN : (double)hv;
     } else if (BK == BK_ARRAY) {                           // array_depth.rs:26-33
         // Rust `as usize`: saturating, NaN -> 0
-        uint64_t xi = px > 0.0f ? (px >= 1.8446744e19f ? UINT64_MAX : (uint64_t)px) : 0;
-        uint64_t yi = py > 0.0f ? (py >= 1.8446744e19f ? UINT64_MAX : (uint64_t)py) : 0;
-        if (xi >= (uint64_t)B->an || yi >= (uint64_t)B->an) {
-            h = hx = hy = MR_NAN;
-            return true;
-        }
-        h = (double)B->arr[xi * (uint64_t)B->an + yi];
+        const uint64_t xi = px > 0.0f ? (px >= 1.8446744e19f ? UINT64_MAX : (uint64_t)px) : 0;
+        const uint64_t yi = py > 0.0f ? (py >= 1.8446744e19f ? UINT64_MAX : (uint64_t)py) : 0;
         hx = 0.0; hy = 0.0;
-        return true;
+        if (xi >= (uint64_t)B.an || yi >= (uint64_t)B.an) return MR_NAN;
+        return (double)gld(P.arr, (uint32_t)(xi * (uint64_t)B.an + yi));
     } else {                                               // cartesian_netcdf3.rs:98-135
-        // predicated: every load is in bounds, errors only clear ok
-        const bool nan_pt = __builtin_isnan(px) || __builtin_isnan(py);
-        const float ix = index32(px - B->ax0, B->inv_sx);  // nearest :274-296
-        const float iy = index32(py - B->ay0, B->inv_sy);
-        bool ok = ix >= 0.0f && ix <= B->nxm1 && iy >= 0.0f && iy <= B->nym1;
-        // four_corners :334-390: floor, or n-2 on the last node
-        const int i1 = (int)__builtin_fminf(__builtin_fmaxf(ix, 0.0f), B->nxm2);
-        const int j1 = (int)__builtin_fminf(__builtin_fmaxf(iy, 0.0f), B->nym2);
-        int64_t cell = (int64_t)j1 * (B->nx - 1) + i1;
-        ok = ok && cell < B->oob_from;
-        cell = ok ? cell : 0;
-        const float4 *rec = reinterpret_cast<const float4 *>(B->cells + cell);
-        const float4 r0 = rec[0], r1 = rec[1];
-        const float2 xp = B->xp[i1];                      // x[i1], x[i1+1]
-        const float2 yq = B->yp[j1];
-        const float ya = yq.x, yb = yq.y;
-        float hv = bilin_eval(r0.x, r0.y, r0.z, r0.w, r1.z * (py - ya), r1.w * (px - xp.x));
-        const bool corner = (px == xp.x || px == xp.y) && (py == ya || py == yb);
-        if (corner && ok)
-            hv = (float)B->raw[corner_index(px, py, xp.x, xp.y, ya, yb, B->nx, i1, j1)];
-        ok = ok && (corner || !__builtin_isnan(r1.z));     // det == 0 errs
-        h = nan_pt ? MR_NAN : (double)hv;
-        hx = nan_pt ? MR_NAN : (double)r1.x;
-        hy = nan_pt ? MR_NAN : (double)r1.y;
-        return ok || nan_pt;
+        // still in the cached cell: no index arithmetic (a NaN target fails
+        // the comparisons)
+        bool ok = bathy_inside(bc.cur, px, py);
+        if (__any(!ok)) {
+            const bool miss = !ok;
+            bool okn = false;
+            {
+                // nearest (:274-296); a NaN target fails every comparison
+                // and yields NaN, as the reference's early return does
+                const float ix = index32(px - B.ax0, B.inv_sx);
+                const float iy = index32(py - B.ay0, B.inv_sy);
+                okn = (ix >= 0.0f) & (ix <= B.nxm1) & (iy >= 0.0f) & (iy <= B.nym1);
+                // four_corners (:334-390): floor, or n-2 on the last node
+                const int i1 = __builtin_elementwise_min((int)ix, B.nx - 2);
+                const int j1 = __builtin_elementwise_min((int)iy, B.ny - 2);
+                okn = okn & ((uint32_t)j1 * (uint32_t)B.ncx + (uint32_t)i1 < B.oob_from);
+                if (miss & okn) bathy_load(B, P, bc.cur, i1, j1);
+            }
+            ok = ok | (miss & okn);
+        }
+        const float dxa = px - bc.cur.ax.x, dya = py - bc.cur.ay.x;
+        const float4 r0 = bc.cur.r0, r1 = bc.cur.r1;
+        float hv = bilin_eval(r0.x, r0.y, r0.z, r0.w, r1.z * dya, r1.w * dxa);
+        const float dxd = px - bc.cur.ax.y, dyb = py - bc.cur.ay.y;
+        const bool corner = on_corner(dxa, dxd, dya, dyb);
+        if (corner & ok)
+            hv = (float)gld(P.raw, corner_index(dxa, dya, dyb, B.nx, bc.cur.i1, bc.cur.j1));
+        ok = ok & (corner | !__builtin_isnan(r1.z));       // det == 0 errs
+        hx = (double)r1.x;
+        hy = (double)r1.y;
+        return ok ? (double)hv : MR_NAN;
     }
 }
 
 // ---------------------------------------------------------------------------
-// current: CurrentData::current_and_gradient (current/mod.rs:21-31)
+// current: CurrentData::current_and_gradient (current/mod.rs:21-31); returns
+// false where the reference errs.
+
+__device__ __forceinline__ bool current_inside(const CurrentSlot &s, double x, double y)
+{
+    return (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
+}
+
+__device__ __forceinline__ void current_load(const CurrentView &C, const CurrentPtrs &P,
+                                             CurrentSlot &s, int i1, int j1)
+{
+    const uint32_t cell = tiled_cell(i1, j1, C.ntx);
+    s.r0 = gld4(P.cells, cell * 64u);
+    s.r1 = gld4(P.cells, cell * 64u + 16u);
+    s.g0 = gldd2(P.cells, cell * 64u + 32u);
+    s.g1 = gldd2(P.cells, cell * 64u + 48u);
+    s.xp = gld2(P.xp, (uint32_t)i1);
+    s.yp = gld2(P.yp, (uint32_t)j1);
+    s.tx = gldd2(P.xt, (uint32_t)i1 * 16u);
+    s.ty = gldd2(P.yt, (uint32_t)j1 * 16u);
+    s.m = P.mtab ? gld2(P.mtab, cell) : make_float2(C.m01, C.m10);
+    s.i1 = i1;
+    s.j1 = j1;
+}
 
 template <int CK>
-__device__ __forceinline__ bool current_query(const MR_CONST CurrentView *C, double x, double y,
-                                              double &u, double &v, double &ux, double &uy,
-                                              double &vx, double &vy)
+__device__ __forceinline__ bool current_query(const CurrentView &C, const CurrentPtrs &P,
+                                              CurrentCache &cc, double x, double y, double &u,
+                                              double &v, double &ux, double &uy, double &vxg,
+                                              double &vyg)
 {
     if (CK == CK_CONSTANT) {                               // constant_current.rs:66-76
-        u = C->u; v = C->v;
-        ux = 0.0; uy = 0.0; vx = 0.0; vy = 0.0;
+        u = C.u; v = C.v;
+        ux = 0.0; uy = 0.0; vxg = 0.0; vyg = 0.0;
         return true;
     } else {                                               // cartesian_current.rs:487-542
-        bool ok = true;
-        const int i1 = index64(x, C->cx0, C->sx, C->inv_sx, C->nxm1, C->nxm2, ok);
-        const int j1 = index64(y, C->cy0, C->sy, C->inv_sy, C->nym1, C->nym2, ok);
-        int64_t cell = (int64_t)j1 * (C->nx - 1) + i1;
-        ok = ok && cell < C->oob_from;
-        cell = ok ? cell : 0;
-        const float4 *rec = reinterpret_cast<const float4 *>(C->cells + cell);
-        const float4 r0 = rec[0], r1 = rec[1];
-        const double2 g0 = reinterpret_cast<const double2 *>(rec + 2)[0];
-        const double2 g1 = reinterpret_cast<const double2 *>(rec + 2)[1];
-        float m01 = C->m01, m10 = C->m10;
-        if (C->mtab) {
-            const float2 m = C->mtab[cell];
-            m01 = m.x; m10 = m.y;
+        // no NaN guard (:492-495): a NaN target fails the comparisons and
+        // then errs, as the reference does
+        bool ok = current_inside(cc.cur, x, y);
+        if (__any(!ok)) {
+            const bool miss = !ok;
+            bool okn = false;
+            {
+                okn = true;
+                const double fx = index64(x, C.cx0, C.sx, C.inv_sx, C.nxm1, okn);
+                const double fy = index64(y, C.cy0, C.sy, C.inv_sy, C.nym1, okn);
+                const int i1 = __builtin_elementwise_min((int)fx, C.nx - 2);
+                const int j1 = __builtin_elementwise_min((int)fy, C.ny - 2);
+                okn = okn & ((uint32_t)j1 * (uint32_t)C.ncx + (uint32_t)i1 < C.oob_from);
+                if (miss & okn) current_load(C, P, cc.cur, i1, j1);
+            }
+            ok = ok | (miss & okn);
         }
         const float tx = (float)x, ty = (float)y;          // cartesian_current.rs:505-512
-        const float2 xp = C->xp[i1];
-        const float2 yq = C->yp[j1];
-        const float ya = yq.x, yb = yq.y;
-        const float uw = m01 * (ty - ya);
-        const float ww = m10 * (tx - xp.x);
+        const float dxa = tx - cc.cur.xp.x, dya = ty - cc.cur.yp.x;
+        const float uw = cc.cur.m.x * dya;
+        const float ww = cc.cur.m.y * dxa;
+        const float4 r0 = cc.cur.r0, r1 = cc.cur.r1;
         float uf = bilin_eval(r0.x, r0.y, r0.z, r0.w, uw, ww);
         float vf = bilin_eval(r1.x, r1.y, r1.z, r1.w, uw, ww);
-        const bool corner = (tx == xp.x || tx == xp.y) && (ty == ya || ty == yb);
-        if (corner && ok) {
-            const int64_t ic = corner_index(tx, ty, xp.x, xp.y, ya, yb, C->nx, i1, j1);
-            uf = (float)C->raw_u[ic];
-            vf = (float)C->raw_v[ic];
+        const float dxd = tx - cc.cur.xp.y, dyb = ty - cc.cur.yp.y;
+        const bool corner = on_corner(dxa, dxd, dya, dyb);
+        if (corner & ok) {
+            const uint32_t ic = corner_index(dxa, dya, dyb, C.nx, cc.cur.i1, cc.cur.j1);
+            uf = (float)gld(P.raw_u, ic);
+            vf = (float)gld(P.raw_v, ic);
         }
-        ok = ok && (corner || !__builtin_isnan(m01));       // det == 0 errs
+        ok = ok & (corner | !__builtin_isnan(cc.cur.m.x));  // det == 0 errs
         u = (double)uf;
         v = (double)vf;
-        ux = g0.x; uy = g0.y; vx = g1.x; vy = g1.y;
+        ux = cc.cur.g0.x; uy = cc.cur.g0.y; vxg = cc.cur.g1.x; vyg = cc.cur.g1.y;
         return ok;
     }
 }
@@ -284,17 +422,18 @@ __device__ __forceinline__ double rsq_nr(double x)
     return __builtin_fma(0.5 * y, e, y);
 }
 
-// e = exp(-r) and em = 1 - exp(-r) for r >= 0, each with small relative
-// error, from one reduced-argument evaluation: r = n ln2 + f, |f| <= ln2/2,
-// expm1(-f) by its Taylor series through degree 13 (truncation below
-// 2^-56 relative on that interval), e = 2^-n (1 + p), em = (1 - 2^-n) -
-// 2^-n p, which has no cancellation for n >= 1 and is exactly -p for n = 0.
+// e = exp(-r) and em = 1 - exp(-r), each with small relative error, from one
+// reduced-argument evaluation: r = n ln2 + f, |f| <= ln2/2, expm1(-f) by its
+// Taylor series through degree 13 (truncation below 2^-56 relative on that
+// interval), e = 2^-n (1 + p), em = (1 - 2^-n) - 2^-n p, which has no
+// cancellation for n >= 1 and is exactly -p for n = 0.  |r| is clamped to
+// 1400, beyond which exp(-r) is 0 or overflows in f64 either way.
 __device__ __forceinline__ void exp_pair(double r, double &e, double &em)
 {
     const double INV_LN2 = 1.4426950408889634;          // 1/ln 2
     const double LN2_HI = 6.93147180369123816490e-01;  // ln 2 split, upper part
     const double LN2_LO = 1.90821492927058770002e-10;  // ln 2 - LN2_HI
-    r = r < 1400.0 ? r : 1400.0;                        // exp(-1400) underflows to 0
+    r = __builtin_fmin(__builtin_fmax(r, -1400.0), 1400.0);
     const double n = __builtin_rint(r * INV_LN2);
     double f = __builtin_fma(-n, LN2_HI, r);
     f = __builtin_fma(-n, LN2_LO, f);
@@ -323,15 +462,19 @@ __device__ __forceinline__ void exp_pair(double r, double &e, double &em)
 // WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150)
 
 template <int BK, int CK, int MATH>
-__device__ __forceinline__ void rhs(const MR_CONST BathyView *B, const MR_CONST CurrentView *C, double x,
-                                    double y, double kx, double ky, double &dx, double &dy,
-                                    double &dkx, double &dky)
+__device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathyCache &bc,
+                                    const CurrentView &C, const CurrentPtrs &CP,
+                                    CurrentCache &cc, double x, double y, double kx, double ky,
+                                    double &dx, double &dy, double &dkx, double &dky)
 {
-    double h, hx, hy, u, v, ux, uy, vx, vy;
-    bool ok = bathy_query<BK>(B, x, y, h, hx, hy);         // odes :120-127
-    ok = current_query<CK>(C, x, y, u, v, ux, uy, vx, vy) && ok;   // :130
-
+    double hx, hy, u, v, ux, uy, vx, vy;
+    double h = bathy_query<BK>(B, BP, bc, x, y, hx, hy);             // odes :120-127
+    const bool okc = current_query<CK>(C, CP, cc, x, y, u, v, ux, uy, vx, vy);   // :130
     if (MATH == MATH_LITERAL) {
+        // A NaN depth (a provider error, or a NaN target or datum) makes
+        // every derivative NaN through group_velocity and dkdt_bathy, as an
+        // error does, except when k <= 0 errs first; both end all-NaN.
+        bool ok = okc;
         const double k = sqrt(kx * kx + ky * ky);           // :133-134
         const double theta = atan2(ky, kx);
         // group_velocity :177-188: h <= 0 -> NaN (Ok); k <= 0 -> Err
@@ -354,14 +497,17 @@ __device__ __forceinline__ void rhs(const MR_CONST BathyView *B, const MR_CONST 
         dky = ok ? (dkyb - kx * uy) - ky * vy : MR_NAN;
     } else {
         // Every provider error, h <= 0 (or NaN), k <= 0 (or NaN) and a
-        // non-finite k*h give all-NaN derivatives in the reference (an error
-        // directly; the others through NaN/inf propagation in
-        // group_velocity and dkdt_bathy), so one mask covers them.
-        const double k2 = kx * kx + ky * ky;
+        // non-finite k*h give all-NaN derivatives in the reference (an
+        // error directly; the others through NaN/inf propagation in
+        // group_velocity and dkdt_bathy).  Here a provider error poisons h
+        // with NaN, and each of these cases makes cg and c NaN: k = 0 gives
+        // k = 0 * rsq(0) = NaN; h <= 0 gives tanh(kh) <= 0 and a NaN
+        // rsq(g k tanh); a NaN or infinite kh propagates.
+        h = okc ? h : MR_NAN;
+        const double k2 = __builtin_fma(kx, kx, ky * ky);
         const double rk = rsq_nr(k2);
         const double k = k2 * rk;
         const double kh = k * h;
-        const bool bad = !ok || !(h > 0.0) || !(k > 0.0) || !(kh < __builtin_inf());
         const double cth = kx * rk, sth = ky * rk;          // cos, sin of atan2(ky, kx)
         double e, em;
         exp_pair(2.0 * kh, e, em);                          // e = exp(-2kh)
@@ -373,13 +519,13 @@ __device__ __forceinline__ void rhs(const MR_CONST BathyView *B, const MR_CONST 
         const double gkT = (G * k) * T;
         const double rs = rsq_nr(gkT);
         const double sigma = gkT * rs;                      // sqrt(g k tanh kh)
-        const double cg = (0.5 * G) * (T + kh * sech2) * rs;
+        const double cg = (0.5 * G) * __builtin_fma(kh, sech2, T) * rs;
         // (-k/2)/(sinh cosh) * sigma = -2 k e sigma / (em (1+e))
         const double c = (-2.0 * k) * e * sigma * D;
-        dx = bad ? MR_NAN : cg * cth + u;
-        dy = bad ? MR_NAN : cg * sth + v;
-        dkx = bad ? MR_NAN : (c * hx - kx * ux) - ky * vx;
-        dky = bad ? MR_NAN : (c * hy - kx * uy) - ky * vy;
+        dx = __builtin_fma(cg, cth, u);
+        dy = __builtin_fma(cg, sth, v);
+        dkx = (c * hx - kx * ux) - ky * vx;
+        dky = (c * hy - kx * uy) - ky * vy;
     }
 }
 

@@ -5,14 +5,17 @@
 
 namespace mr {
 
-// The field views live in device memory and are re-read (scalar loads) at
-// each right-hand-side evaluation; holding all of them in SGPRs across the
-// step loop would exceed the SGPR file and spill to VGPR lanes.
+// The numeric field views live in device memory and are copied into
+// per-lane registers at kernel start (see load_view); the field pointers
+// travel as kernel arguments.
 struct TraceArgs {
     const void *B;                     // BathyView in device memory
     const void *C;                     // CurrentView in device memory
+    BathyPtrs bp;
+    CurrentPtrs cp;
     int64_t n_rays, n_steps;
-    double dt, half, sixth;
+    double t0, dt, half, sixth;
+    double *t_out;                     // n_steps + 1 times, may be null
     const double *x0, *y0, *kx0, *ky0;
     double *xo, *yo, *kxo, *kyo;       // step-major, may be null
     int32_t *len_out;                  // may be null
@@ -23,6 +26,8 @@ struct TraceArgs {
 struct RhsArgs {
     const void *B;
     const void *C;
+    BathyPtrs bp;
+    CurrentPtrs cp;
     int64_t n;
     const double *s;
     double *ds;
@@ -32,8 +37,9 @@ struct BathyBuildArgs {
     int64_t nx, ny;
     const float2 *xp, *yp;
     const double *raw;
-    int64_t oob_from;
+    int64_t oob_from;                  // cells from here on are never read
     double xspace, yspace;
+    int ntx;                           // 8 x 8 record tiles along x
     BCell *cells;
 };
 
@@ -43,6 +49,7 @@ struct CurrentBuildArgs {
     const double *u, *v;
     int64_t oob_from;
     double xspace, yspace;
+    int ntx;
     CCell *cells;
     float2 *mtab;                      // null when the f32 axes are uniform
 };

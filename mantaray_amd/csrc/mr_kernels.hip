@@ -12,13 +12,24 @@
 
 namespace mr {
 
-__device__ __forceinline__ const MR_CONST BathyView *bview(const void *p)
+// Copy a field view from device memory into per-lane registers.  The loads
+// go through a lane-varying (always zero) offset so they are vector loads
+// and the view lives in VGPRs for the whole kernel: the scalar register file
+// cannot hold both views next to the loop's own scalars, and re-reading
+// them with scalar loads puts a load latency into every evaluation.
+template <class View>
+__device__ __forceinline__ View load_view(const void *p)
 {
-    return (const MR_CONST BathyView *)p;
-}
-__device__ __forceinline__ const MR_CONST CurrentView *cview(const void *p)
-{
-    return (const MR_CONST CurrentView *)p;
+    static_assert(sizeof(View) % 4 == 0, "view size");
+    int zero;
+    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    const int *src = reinterpret_cast<const int *>(p) + zero;
+    int buf[sizeof(View) / 4];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(View) / 4); ++i) buf[i] = src[i];
+    View v;
+    __builtin_memcpy(&v, buf, sizeof(View));
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -40,22 +51,32 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (a.kxo) a.kxo[r] = kx;
         if (a.kyo) a.kyo[r] = ky;
     }
+    const BathyView B = load_view<BathyView>(a.B);
+    const CurrentView C = load_view<CurrentView>(a.C);
+    BathyCache bc;
+    CurrentCache cc;
     bool alive = active;
     int64_t len = active ? 1 : 0;
     const double half = a.half, dt = a.dt, sixth = a.sixth;
+    // the time column t_s = t_{s-1} + dt (ode_solvers accumulates), written
+    // by the first lane of the grid alongside its own rows
+    const bool timekeeper = a.t_out && r == 0;
+    double tcur = a.t0;
+    if (timekeeper) a.t_out[0] = tcur;
     int64_t s = 1;
     for (; s <= a.n_steps; ++s) {
+        tcur = tcur + dt;
+        if (timekeeper) a.t_out[s] = tcur;
         if (!__any(alive)) break;
         double ox = MR_NAN, oy = MR_NAN, okx = MR_NAN, oky = MR_NAN;
         if (alive) {
             double sx = x, sy = y, skx = kx, sky = ky;   // stage state
             double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
             bool stop_before = false;
-#pragma unroll 1
+#pragma unroll
             for (int st = 0; st < 4; ++st) {
                 double dx, dy, dkx, dky;
-                __asm__ volatile("" ::: "memory");   // re-read the views here
-                rhs<BK, CK, MATH>(bview(a.B), cview(a.C), sx, sy, skx, sky, dx, dy, dkx, dky);
+                rhs<BK, CK, MATH>(B, a.bp, bc, C, a.cp, cc, sx, sy, skx, sky, dx, dy, dkx, dky);
                 if (st == 0) {
                     // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
                     // just pushed: an all-NaN k1 here means the ray already
@@ -65,10 +86,12 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                         break;
                     }
                     ax = dx; ay = dy; akx = dkx; aky = dky;
+                } else if (st < 3) {
+                    // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
+                    ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
+                    akx = __builtin_fma(dkx, 2.0, akx); aky = __builtin_fma(dky, 2.0, aky);
                 } else {
-                    const double w = st == 3 ? 1.0 : 2.0;
-                    ax = ax + dx * w; ay = ay + dy * w;
-                    akx = akx + dkx * w; aky = aky + dky * w;
+                    ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
                 }
                 const double hs = st < 2 ? half : dt;
                 sx = x + dx * hs; sy = y + dy * hs;
@@ -93,7 +116,14 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         }
     }
     if (active) {
+        // rows past the wave's last live step (its time entry is written)
+        bool first = true;
         for (; s <= a.n_steps; ++s) {
+            if (!first) {
+                tcur = tcur + dt;
+                if (timekeeper) a.t_out[s] = tcur;
+            }
+            first = false;
             const int64_t o = s * nr + r;
             if (a.xo) a.xo[o] = MR_NAN;
             if (a.yo) a.yo[o] = MR_NAN;
@@ -114,7 +144,11 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     double dx, dy, dkx, dky;
-    rhs<BK, CK, MATH>(bview(a.B), cview(a.C), a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
+    const BathyView B = load_view<BathyView>(a.B);
+    const CurrentView C = load_view<CurrentView>(a.C);
+    BathyCache bc;
+    CurrentCache cc;
+    rhs<BK, CK, MATH>(B, a.bp, bc, C, a.cp, cc, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
                       dx, dy, dkx, dky);
     a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
 }
@@ -172,10 +206,10 @@ __global__ void build_bathy_kernel(BathyBuildArgs a)
         r.hx = (float)((a.raw[iD] - a.raw[iA]) / a.xspace);  // cartesian_netcdf3.rs:119-134
         r.hy = (float)((a.raw[iB] - a.raw[iA]) / a.yspace);
     }
-    const float2 m = cell_basis(a.xp, a.yp, i, j);
+    const float2 m = cell_basis(a.xp, a.yp, i, j);   // m01 = NaN marks det == 0
     r.m01 = m.x;
     r.m10 = m.y;
-    a.cells[c] = r;
+    a.cells[tiled_cell((int)i, (int)j, a.ntx)] = r;
 }
 
 __global__ void build_current_kernel(CurrentBuildArgs a)
@@ -201,8 +235,9 @@ __global__ void build_current_kernel(CurrentBuildArgs a)
         r.vx = (a.v[iD] - a.v[iA]) / a.xspace;
         r.vy = (a.v[iB] - a.v[iA]) / a.yspace;
     }
-    a.cells[c] = r;
-    if (a.mtab) a.mtab[c] = cell_basis(a.xp, a.yp, i, j);
+    const uint32_t t = tiled_cell((int)i, (int)j, a.ntx);
+    a.cells[t] = r;
+    if (a.mtab) a.mtab[t] = cell_basis(a.xp, a.yp, i, j);
 }
 
 // ---------------------------------------------------------------------------
