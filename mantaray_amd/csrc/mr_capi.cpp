@@ -297,6 +297,10 @@ int build_bathy(const mr_field &f, DevTables &t)
         v.nym1 = (float)(f.ny - 1);
         v.inv_sx = 1.0 / (double)std::fabs(f.xf[1] - f.xf[0]);   // spacing in f32
         v.inv_sy = 1.0 / (double)std::fabs(f.yf[1] - f.yf[0]);
+        v.pinv_sx = (float)v.inv_sx;
+        v.pinv_sy = (float)v.inv_sy;
+        v.pmax_x = (float)(f.nx - 2);
+        v.pmax_y = (float)(f.ny - 2);
         std::vector<float2> xp = axis_pairs(f.xf), yp = axis_pairs(f.yf);
         std::vector<float4> xa = axis_cells32(f.xf), ya = axis_cells32(f.yf);
         float2 *dxp, *dyp;
@@ -356,24 +360,15 @@ int build_current(const mr_field &f, DevTables &t)
         v.inv_sy = 1.0 / v.sy;
         v.nxm1 = (double)(f.nx - 1);
         v.nym1 = (double)(f.ny - 1);
+        v.px0 = (float)v.cx0;
+        v.py0 = (float)v.cy0;
+        v.pinv_sx = (float)v.inv_sx;
+        v.pinv_sy = (float)v.inv_sy;
+        v.pmax_x = (float)(f.nx - 2);
+        v.pmax_y = (float)(f.ny - 2);
         std::vector<float> x32(f.xd.size()), y32(f.yd.size());
         for (size_t i = 0; i < x32.size(); i++) x32[i] = (float)f.xd[i];   // :375-378
         for (size_t i = 0; i < y32.size(); i++) y32[i] = (float)f.yd[i];
-        // uniform f32 axes: one change of basis for every cell
-        bool uniform = true;
-        const float dx0 = x32[1] - x32[0], dy0 = y32[1] - y32[0];
-        for (size_t i = 0; i + 1 < x32.size() && uniform; i++) uniform = (x32[i + 1] - x32[i]) == dx0;
-        for (size_t i = 0; i + 1 < y32.size() && uniform; i++) uniform = (y32[i + 1] - y32[i]) == dy0;
-        if (uniform) {
-            const float btx = x32[0] - x32[0], bty = dy0, dtx = dx0, dty = y32[0] - y32[0];
-            const float det = (btx * dty) - (dtx * bty);       // interpolator.rs:64
-            if (det == 0.0f) {
-                uniform = false;
-            } else {
-                v.m01 = -(dtx / det);
-                v.m10 = -(bty / det);
-            }
-        }
         std::vector<float2> xp = axis_pairs(x32), yp = axis_pairs(y32);
         std::vector<double2> xt = axis_cells64(f.xd), yt = axis_cells64(f.yd);
         float2 *dxp, *dyp;
@@ -389,9 +384,9 @@ int build_current(const mr_field &f, DevTables &t)
         const int ntx = (int)((f.nx - 1 + 7) / 8), nty = (int)((f.ny - 1 + 7) / 8);
         const int64_t ntile_cells = (int64_t)ntx * nty * 64;     // 8 x 8-cell tiles
         CCell *cells;
-        float2 *mtab = nullptr;
+        float2 *mtab;
         rc = dev_alloc(t, (size_t)ntile_cells, &cells);
-        if (!rc && !uniform) rc = dev_alloc(t, (size_t)ntile_cells, &mtab);
+        if (!rc) rc = dev_alloc(t, (size_t)ntile_cells, &mtab);
         if (rc) return rc;
         CurrentBuildArgs ca;
         ca.nx = f.nx; ca.ny = f.ny; ca.xp = dxp; ca.yp = dyp; ca.u = du; ca.v = dv;
@@ -486,7 +481,7 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     int rc = field_tables(bathy, &bt);
     if (!rc) rc = field_tables(current, &ct);
     if (rc) return rc;
-    if (t_out && n_rays == 0) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
+    if (t_out) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
     TraceArgs a;
     a.B = bt->view;
     a.C = ct->view;
@@ -495,7 +490,6 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.n_rays = n_rays;
     a.n_steps = n;
     a.t0 = t0;
-    a.t_out = t_out;
     a.dt = dt;
     a.half = dt / 2.0;                                 // Rk4::new half_step
     a.sixth = dt / 6.0;                                // Rk4::step (step_size / 6)

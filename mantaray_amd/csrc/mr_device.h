@@ -62,6 +62,8 @@ struct BathyView {
     double inv_sx, inv_sy;                // RN64(1/(double)|a1-a0|)
     uint32_t oob_from;                    // first cell (row-major) with a corner past the data
     int ntx;                              // 8 x 8 record tiles along x
+    float pinv_sx, pinv_sy;               // cell guess for prefetch: (f32) inv_s
+    float pmax_x, pmax_y;                 // (f32)(n - 2)
 };
 
 struct BathyPtrs {
@@ -77,35 +79,37 @@ struct CurrentView {
     double u, v;                          // ConstantCurrent
     double cx0, cy0, sx, sy, inv_sx, inv_sy;
     double nxm1, nym1;                    // n - 1
-    float m01, m10;                       // change of basis of uniform axes
     uint32_t oob_from;                    // first cell (row-major) with a corner past u or v
     int ntx;                              // 8 x 8 record tiles along x
+    float px0, py0, pinv_sx, pinv_sy;     // cell guess for prefetch, in f32
+    float pmax_x, pmax_y;                 // (f32)(n - 2)
 };
 
 struct CurrentPtrs {
     const float2 *xp, *yp;                // pairs of the (f32) casts of the f64 axes
     const double2 *xt, *yt;               // per axis cell: (T[i], T[i+1]) in f64
     const CCell *cells;
-    const float2 *mtab;                   // per-cell (m01, m10), or null if uniform
+    const float2 *mtab;                   // per-cell (m01, m10)
     const double *raw_u, *raw_v;          // raw values (exact-corner values)
 };
 
-// Per-lane copy of the last cell record read, with the cell's index range.
-// Along each axis the cell the reference chooses is a monotone step function
-// of the coordinate (cartesian_netcdf3.rs:274-390, cartesian_current.rs:231-339
+// Per-lane copy of a cell record, with the cell's index range.  Along each
+// axis the cell the reference chooses is a monotone step function of the
+// coordinate (cartesian_netcdf3.rs:274-390, cartesian_current.rs:231-339
 // round, divide and floor monotonically), so cell i is exactly the interval
 // T[i] <= p < T[i+1] of a per-axis threshold table T (T[0] the first valid
 // coordinate, T[n-1] the first one past the last node).  The thresholds are
 // found on the host by bisection over the floating-point values with the
-// reference's own arithmetic.  A ray stays in one cell for many evaluations;
-// while it does, the query costs four comparisons per field and no index
-// arithmetic or load.  Thresholds start as NaN so that the first query
-// misses.
+// reference's own arithmetic.  A query whose target lies in the slot's cell
+// (four comparisons per field, and the slot valid) needs no index arithmetic
+// and no load of its own.  valid is false for a cell with a corner past the
+// data (an error cell) and for a slot never loaded; thresholds start as NaN.
 struct BathySlot {
     float4 r0 = {0.f, 0.f, 0.f, 0.f}, r1 = {0.f, 0.f, 0.f, 0.f};
     float4 ax = {0.f, 0.f, MR_NANF_INIT, MR_NANF_INIT};   // (x_i, x_i+1, T_i, T_i+1)
     float4 ay = {0.f, 0.f, MR_NANF_INIT, MR_NANF_INIT};
     int i1 = 0, j1 = 0;
+    bool valid = false;
 };
 
 struct CurrentSlot {
@@ -114,14 +118,7 @@ struct CurrentSlot {
     float2 xp = {0.f, 0.f}, yp = {0.f, 0.f}, m = {0.f, 0.f};
     double2 tx = {MR_NAN_INIT, MR_NAN_INIT}, ty = {MR_NAN_INIT, MR_NAN_INIT};
     int i1 = 0, j1 = 0;
-};
-
-struct BathyCache {
-    BathySlot cur;
-};
-
-struct CurrentCache {
-    CurrentSlot cur;
+    bool valid = false;
 };
 
 // Loads from device (global) memory through the global address space, so
@@ -243,13 +240,33 @@ __device__ __forceinline__ double index64(double p, double a0, double sp, double
     return fl;
 }
 
+// Wait for every outstanding vector memory operation (s_waitcnt vmcnt(0),
+// expcnt and lgkmcnt left alone).  Used at the end of a reload path so that
+// the loads it issued are complete on that path: the compiler's own wait
+// would otherwise sit after the join, where, because vmcnt retires in issue
+// order, it also waits for the prefetch and the row stores issued before
+// the reload on the common path that did not reload.
+__device__ __forceinline__ void wait_vmem()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
+// The cell of a target, guessed in f32 and clamped into the grid (a NaN
+// guess clamps to 0): used only to choose which record to prefetch, never
+// for the result; the exact cell test happens when the record is used.
+__device__ __forceinline__ int guess_cell(float t, float a0, float inv_sp, float max_cell)
+{
+    const float f = (t - a0) * inv_sp;
+    return (int)__builtin_fminf(__builtin_fmaxf(f, 0.0f), max_cell);
+}
+
 // ---------------------------------------------------------------------------
 // bathymetry: BathymetryData::depth_and_gradient (bathymetry/mod.rs:35-41);
 // returns the depth widened to f64, NaN where the reference errs.
 
 __device__ __forceinline__ bool bathy_inside(const BathySlot &s, float px, float py)
 {
-    return (px >= s.ax.z) & (px < s.ax.w) & (py >= s.ay.z) & (py < s.ay.w);
+    return s.valid & (px >= s.ax.z) & (px < s.ax.w) & (py >= s.ay.z) & (py < s.ay.w);
 }
 
 __device__ __forceinline__ void bathy_load(const BathyView &B, const BathyPtrs &P, BathySlot &s,
@@ -262,11 +279,21 @@ __device__ __forceinline__ void bathy_load(const BathyView &B, const BathyPtrs &
     s.ay = gld4(P.ya, (uint32_t)j1 * 16u);
     s.i1 = i1;
     s.j1 = j1;
+    s.valid = (uint32_t)j1 * (uint32_t)B.ncx + (uint32_t)i1 < B.oob_from;
+}
+
+// Start loading the record of the cell that (x, y) most likely lies in.
+__device__ __forceinline__ void bathy_prefetch(const BathyView &B, const BathyPtrs &P,
+                                               BathySlot &s, double x, double y)
+{
+    const int i1 = guess_cell((float)x, B.ax0, B.pinv_sx, B.pmax_x);
+    const int j1 = guess_cell((float)y, B.ay0, B.pinv_sy, B.pmax_y);
+    bathy_load(B, P, s, i1, j1);
 }
 
 template <int BK>
 __device__ __forceinline__ double bathy_query(const BathyView &B, const BathyPtrs &P,
-                                              BathyCache &bc, double x, double y, double &hx,
+                                              BathySlot &bs, double x, double y, double &hx,
                                               double &hy)
 {
     const float px = (float)x, py = (float)y;             // wave_ray_path.rs:122
@@ -287,33 +314,31 @@ __device__ __forceinline__ double bathy_query(const BathyView &B, const BathyPtr
         if (xi >= (uint64_t)B.an || yi >= (uint64_t)B.an) return MR_NAN;
         return (double)gld(P.arr, (uint32_t)(xi * (uint64_t)B.an + yi));
     } else {                                               // cartesian_netcdf3.rs:98-135
-        // still in the cached cell: no index arithmetic (a NaN target fails
-        // the comparisons)
-        bool ok = bathy_inside(bc.cur, px, py);
+        // in the slot's cell: no index arithmetic (a NaN target fails the
+        // comparisons)
+        bool ok = bathy_inside(bs, px, py);
         if (__any(!ok)) {
             const bool miss = !ok;
-            bool okn = false;
-            {
-                // nearest (:274-296); a NaN target fails every comparison
-                // and yields NaN, as the reference's early return does
-                const float ix = index32(px - B.ax0, B.inv_sx);
-                const float iy = index32(py - B.ay0, B.inv_sy);
-                okn = (ix >= 0.0f) & (ix <= B.nxm1) & (iy >= 0.0f) & (iy <= B.nym1);
-                // four_corners (:334-390): floor, or n-2 on the last node
-                const int i1 = __builtin_elementwise_min((int)ix, B.nx - 2);
-                const int j1 = __builtin_elementwise_min((int)iy, B.ny - 2);
-                okn = okn & ((uint32_t)j1 * (uint32_t)B.ncx + (uint32_t)i1 < B.oob_from);
-                if (miss & okn) bathy_load(B, P, bc.cur, i1, j1);
-            }
+            // nearest (:274-296); a NaN target fails every comparison and
+            // yields NaN, as the reference's early return does
+            const float ix = index32(px - B.ax0, B.inv_sx);
+            const float iy = index32(py - B.ay0, B.inv_sy);
+            bool okn = (ix >= 0.0f) & (ix <= B.nxm1) & (iy >= 0.0f) & (iy <= B.nym1);
+            // four_corners (:334-390): floor, or n-2 on the last node
+            const int i1 = __builtin_elementwise_min((int)ix, B.nx - 2);
+            const int j1 = __builtin_elementwise_min((int)iy, B.ny - 2);
+            okn = okn & ((uint32_t)j1 * (uint32_t)B.ncx + (uint32_t)i1 < B.oob_from);
+            if (miss & okn) bathy_load(B, P, bs, i1, j1);
+            wait_vmem();
             ok = ok | (miss & okn);
         }
-        const float dxa = px - bc.cur.ax.x, dya = py - bc.cur.ay.x;
-        const float4 r0 = bc.cur.r0, r1 = bc.cur.r1;
+        const float dxa = px - bs.ax.x, dya = py - bs.ay.x;
+        const float4 r0 = bs.r0, r1 = bs.r1;
         float hv = bilin_eval(r0.x, r0.y, r0.z, r0.w, r1.z * dya, r1.w * dxa);
-        const float dxd = px - bc.cur.ax.y, dyb = py - bc.cur.ay.y;
+        const float dxd = px - bs.ax.y, dyb = py - bs.ay.y;
         const bool corner = on_corner(dxa, dxd, dya, dyb);
         if (corner & ok)
-            hv = (float)gld(P.raw, corner_index(dxa, dya, dyb, B.nx, bc.cur.i1, bc.cur.j1));
+            hv = (float)gld(P.raw, corner_index(dxa, dya, dyb, B.nx, bs.i1, bs.j1));
         ok = ok & (corner | !__builtin_isnan(r1.z));       // det == 0 errs
         hx = (double)r1.x;
         hy = (double)r1.y;
@@ -327,7 +352,7 @@ __device__ __forceinline__ double bathy_query(const BathyView &B, const BathyPtr
 
 __device__ __forceinline__ bool current_inside(const CurrentSlot &s, double x, double y)
 {
-    return (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
+    return s.valid & (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
 }
 
 __device__ __forceinline__ void current_load(const CurrentView &C, const CurrentPtrs &P,
@@ -342,14 +367,23 @@ __device__ __forceinline__ void current_load(const CurrentView &C, const Current
     s.yp = gld2(P.yp, (uint32_t)j1);
     s.tx = gldd2(P.xt, (uint32_t)i1 * 16u);
     s.ty = gldd2(P.yt, (uint32_t)j1 * 16u);
-    s.m = P.mtab ? gld2(P.mtab, cell) : make_float2(C.m01, C.m10);
+    s.m = gld2(P.mtab, cell);
     s.i1 = i1;
     s.j1 = j1;
+    s.valid = (uint32_t)j1 * (uint32_t)C.ncx + (uint32_t)i1 < C.oob_from;
+}
+
+__device__ __forceinline__ void current_prefetch(const CurrentView &C, const CurrentPtrs &P,
+                                                 CurrentSlot &s, double x, double y)
+{
+    const int i1 = guess_cell((float)x, C.px0, C.pinv_sx, C.pmax_x);
+    const int j1 = guess_cell((float)y, C.py0, C.pinv_sy, C.pmax_y);
+    current_load(C, P, s, i1, j1);
 }
 
 template <int CK>
 __device__ __forceinline__ bool current_query(const CurrentView &C, const CurrentPtrs &P,
-                                              CurrentCache &cc, double x, double y, double &u,
+                                              CurrentSlot &cs, double x, double y, double &u,
                                               double &v, double &ux, double &uy, double &vxg,
                                               double &vyg)
 {
@@ -360,39 +394,37 @@ __device__ __forceinline__ bool current_query(const CurrentView &C, const Curren
     } else {                                               // cartesian_current.rs:487-542
         // no NaN guard (:492-495): a NaN target fails the comparisons and
         // then errs, as the reference does
-        bool ok = current_inside(cc.cur, x, y);
+        bool ok = current_inside(cs, x, y);
         if (__any(!ok)) {
             const bool miss = !ok;
-            bool okn = false;
-            {
-                okn = true;
-                const double fx = index64(x, C.cx0, C.sx, C.inv_sx, C.nxm1, okn);
-                const double fy = index64(y, C.cy0, C.sy, C.inv_sy, C.nym1, okn);
-                const int i1 = __builtin_elementwise_min((int)fx, C.nx - 2);
-                const int j1 = __builtin_elementwise_min((int)fy, C.ny - 2);
-                okn = okn & ((uint32_t)j1 * (uint32_t)C.ncx + (uint32_t)i1 < C.oob_from);
-                if (miss & okn) current_load(C, P, cc.cur, i1, j1);
-            }
+            bool okn = true;
+            const double fx = index64(x, C.cx0, C.sx, C.inv_sx, C.nxm1, okn);
+            const double fy = index64(y, C.cy0, C.sy, C.inv_sy, C.nym1, okn);
+            const int i1 = __builtin_elementwise_min((int)fx, C.nx - 2);
+            const int j1 = __builtin_elementwise_min((int)fy, C.ny - 2);
+            okn = okn & ((uint32_t)j1 * (uint32_t)C.ncx + (uint32_t)i1 < C.oob_from);
+            if (miss & okn) current_load(C, P, cs, i1, j1);
+            wait_vmem();
             ok = ok | (miss & okn);
         }
         const float tx = (float)x, ty = (float)y;          // cartesian_current.rs:505-512
-        const float dxa = tx - cc.cur.xp.x, dya = ty - cc.cur.yp.x;
-        const float uw = cc.cur.m.x * dya;
-        const float ww = cc.cur.m.y * dxa;
-        const float4 r0 = cc.cur.r0, r1 = cc.cur.r1;
+        const float dxa = tx - cs.xp.x, dya = ty - cs.yp.x;
+        const float uw = cs.m.x * dya;
+        const float ww = cs.m.y * dxa;
+        const float4 r0 = cs.r0, r1 = cs.r1;
         float uf = bilin_eval(r0.x, r0.y, r0.z, r0.w, uw, ww);
         float vf = bilin_eval(r1.x, r1.y, r1.z, r1.w, uw, ww);
-        const float dxd = tx - cc.cur.xp.y, dyb = ty - cc.cur.yp.y;
+        const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
         const bool corner = on_corner(dxa, dxd, dya, dyb);
         if (corner & ok) {
-            const uint32_t ic = corner_index(dxa, dya, dyb, C.nx, cc.cur.i1, cc.cur.j1);
+            const uint32_t ic = corner_index(dxa, dya, dyb, C.nx, cs.i1, cs.j1);
             uf = (float)gld(P.raw_u, ic);
             vf = (float)gld(P.raw_v, ic);
         }
-        ok = ok & (corner | !__builtin_isnan(cc.cur.m.x));  // det == 0 errs
+        ok = ok & (corner | !__builtin_isnan(cs.m.x));      // det == 0 errs
         u = (double)uf;
         v = (double)vf;
-        ux = cc.cur.g0.x; uy = cc.cur.g0.y; vxg = cc.cur.g1.x; vyg = cc.cur.g1.y;
+        ux = cs.g0.x; uy = cs.g0.y; vxg = cs.g1.x; vyg = cs.g1.y;
         return ok;
     }
 }
@@ -462,14 +494,14 @@ __device__ __forceinline__ void exp_pair(double r, double &e, double &em)
 // WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150)
 
 template <int BK, int CK, int MATH>
-__device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathyCache &bc,
+__device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
                                     const CurrentView &C, const CurrentPtrs &CP,
-                                    CurrentCache &cc, double x, double y, double kx, double ky,
+                                    CurrentSlot &cs, double x, double y, double kx, double ky,
                                     double &dx, double &dy, double &dkx, double &dky)
 {
     double hx, hy, u, v, ux, uy, vx, vy;
-    double h = bathy_query<BK>(B, BP, bc, x, y, hx, hy);             // odes :120-127
-    const bool okc = current_query<CK>(C, CP, cc, x, y, u, v, ux, uy, vx, vy);   // :130
+    double h = bathy_query<BK>(B, BP, bs, x, y, hx, hy);             // odes :120-127
+    const bool okc = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);   // :130
     if (MATH == MATH_LITERAL) {
         // A NaN depth (a provider error, or a NaN target or datum) makes
         // every derivative NaN through group_velocity and dkdt_bathy, as an

@@ -15,7 +15,6 @@ struct TraceArgs {
     CurrentPtrs cp;
     int64_t n_rays, n_steps;
     double t0, dt, half, sixth;
-    double *t_out;                     // n_steps + 1 times, may be null
     const double *x0, *y0, *kx0, *ky0;
     double *xo, *yo, *kxo, *kyo;       // step-major, may be null
     int32_t *len_out;                  // may be null
@@ -51,7 +50,7 @@ struct CurrentBuildArgs {
     double xspace, yspace;
     int ntx;
     CCell *cells;
-    float2 *mtab;                      // null when the f32 axes are uniform
+    float2 *mtab;                      // per-cell change of basis
 };
 
 hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,

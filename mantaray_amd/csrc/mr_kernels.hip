@@ -7,8 +7,11 @@
 //   rhs_kernel     WaveRayPath::system (src/wave_ray_path.rs:220-234) at a
 //                  batch of states (parity tests of the right-hand side).
 //   build_*        per-cell records of a gridded field (see mr_device.h).
-//   fill_time      the time column, t += dt accumulated as ode_solvers does.
+//   fill_time      the time column, t += dt accumulated as ode_solvers does
+//                  (one lane: the accumulation is a sequential rounding chain).
 #include "mr_kernels.h"
+
+#include <type_traits>
 
 namespace mr {
 
@@ -37,15 +40,29 @@ __device__ __forceinline__ View load_view(const void *p)
 //   k1 = f(y), k2 = f(y + k1 (dt/2)), k3 = f(y + k2 (dt/2)), k4 = f(y + k3 dt)
 //   y <- y + (((k1 + k2 2) + k3 2) + k4) (dt/6)
 // with the stop test of WaveRayPath::solout (wave_ray_path.rs:236-246).
-template <int BK, int CK, int MATH>
+//
+// Each evaluation of the right-hand side starts the loads of the cell
+// records that the NEXT evaluation will most likely use (its point guessed
+// from the latest derivative) when that is not the cell already held, so a
+// record is in registers one evaluation after it was asked for and a ray
+// crossing into a new cell does not wait for memory; a wrong guess only
+// costs a blocking reload in the query.
+//
+// Lanes past n_rays repeat the last ray (same inputs, same arithmetic, so
+// their stores write the same bytes to the same addresses): every lane
+// stores every row, with no branch around the stores.  OUT: 0 no
+// trajectories, 1 all four components, 2 any subset.
+template <int BK, int CK, int MATH, int OUT>
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = r < a.n_rays;
+    constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
     const int64_t nr = a.n_rays;
-    double x = 0.0, y = 0.0, kx = 0.0, ky = 0.0;
-    if (active) {
-        x = a.x0[r]; y = a.y0[r]; kx = a.kx0[r]; ky = a.ky0[r];
+    const int64_t r = __builtin_elementwise_min((int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                                nr - 1);
+    double x = a.x0[r], y = a.y0[r], kx = a.kx0[r], ky = a.ky0[r];
+    if (OUT == 1) {
+        a.xo[r] = x; a.yo[r] = y; a.kxo[r] = kx; a.kyo[r] = ky;
+    } else if (OUT == 2) {
         if (a.xo) a.xo[r] = x;
         if (a.yo) a.yo[r] = y;
         if (a.kxo) a.kxo[r] = kx;
@@ -53,20 +70,16 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     }
     const BathyView B = load_view<BathyView>(a.B);
     const CurrentView C = load_view<CurrentView>(a.C);
-    BathyCache bc;
-    CurrentCache cc;
-    bool alive = active;
-    int64_t len = active ? 1 : 0;
     const double half = a.half, dt = a.dt, sixth = a.sixth;
-    // the time column t_s = t_{s-1} + dt (ode_solvers accumulates), written
-    // by the first lane of the grid alongside its own rows
-    const bool timekeeper = a.t_out && r == 0;
-    double tcur = a.t0;
-    if (timekeeper) a.t_out[0] = tcur;
+    BathySlot bn;                      // records prefetched for the next evaluation
+    CurrentSlot cn;
+    if (PF_B) bathy_prefetch(B, a.bp, bn, x, y);
+    if (PF_C) current_prefetch(C, a.cp, cn, x, y);
+    double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
+    bool alive = true;
+    int64_t len = 1;
     int64_t s = 1;
     for (; s <= a.n_steps; ++s) {
-        tcur = tcur + dt;
-        if (timekeeper) a.t_out[s] = tcur;
         if (!__any(alive)) break;
         double ox = MR_NAN, oy = MR_NAN, okx = MR_NAN, oky = MR_NAN;
         if (alive) {
@@ -75,16 +88,25 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
             bool stop_before = false;
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
+                const double hs = st < 2 ? half : dt;
+                BathySlot bq = bn;
+                CurrentSlot cq = cn;
+                // the next evaluation is at x + k hs (stages 1-3) or at the
+                // next step's start, about x + k dt; only lanes whose guess
+                // leaves the cell in hand load a record
+                const double gx = x + px * hs, gy = y + py * hs;
+                if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
+                    bathy_prefetch(B, a.bp, bn, gx, gy);
+                if (PF_C && !current_inside(cq, gx, gy))
+                    current_prefetch(C, a.cp, cn, gx, gy);
                 double dx, dy, dkx, dky;
-                rhs<BK, CK, MATH>(B, a.bp, bc, C, a.cp, cc, sx, sy, skx, sky, dx, dy, dkx, dky);
+                rhs<BK, CK, MATH>(B, a.bp, bq, C, a.cp, cq, sx, sy, skx, sky, dx, dy, dkx, dky);
+                px = dx; py = dy;
                 if (st == 0) {
                     // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
                     // just pushed: an all-NaN k1 here means the ray already
                     // ended on that row.
-                    if (a.solout == 1 && all_nan4(dx, dy, dkx, dky)) {
-                        stop_before = true;
-                        break;
-                    }
+                    stop_before = a.solout == 1 && all_nan4(dx, dy, dkx, dky);
                     ax = dx; ay = dy; akx = dkx; aky = dky;
                 } else if (st < 3) {
                     // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
@@ -93,7 +115,6 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                 } else {
                     ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
                 }
-                const double hs = st < 2 ? half : dt;
                 sx = x + dx * hs; sy = y + dy * hs;
                 skx = kx + dkx * hs; sky = ky + dky * hs;
             }
@@ -107,34 +128,30 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                 if (all_nan4(ox, oy, okx, oky)) alive = false;
             }
         }
-        if (active) {
-            const int64_t o = s * nr + r;
+        const int64_t o = s * nr + r;
+        if (OUT == 1) {
+            a.xo[o] = ox; a.yo[o] = oy; a.kxo[o] = okx; a.kyo[o] = oky;
+        } else if (OUT == 2) {
             if (a.xo) a.xo[o] = ox;
             if (a.yo) a.yo[o] = oy;
             if (a.kxo) a.kxo[o] = okx;
             if (a.kyo) a.kyo[o] = oky;
         }
     }
-    if (active) {
-        // rows past the wave's last live step (its time entry is written)
-        bool first = true;
+    // rows past the wave's last live step
+    if (OUT != 0) {
         for (; s <= a.n_steps; ++s) {
-            if (!first) {
-                tcur = tcur + dt;
-                if (timekeeper) a.t_out[s] = tcur;
-            }
-            first = false;
             const int64_t o = s * nr + r;
-            if (a.xo) a.xo[o] = MR_NAN;
-            if (a.yo) a.yo[o] = MR_NAN;
-            if (a.kxo) a.kxo[o] = MR_NAN;
-            if (a.kyo) a.kyo[o] = MR_NAN;
+            if (OUT == 1 || a.xo) a.xo[o] = MR_NAN;
+            if (OUT == 1 || a.yo) a.yo[o] = MR_NAN;
+            if (OUT == 1 || a.kxo) a.kxo[o] = MR_NAN;
+            if (OUT == 1 || a.kyo) a.kyo[o] = MR_NAN;
         }
-        if (a.len_out) a.len_out[r] = (int32_t)len;
-        if (a.end_out) {
-            a.end_out[4 * r + 0] = x; a.end_out[4 * r + 1] = y;
-            a.end_out[4 * r + 2] = kx; a.end_out[4 * r + 3] = ky;
-        }
+    }
+    if (a.len_out) a.len_out[r] = (int32_t)len;
+    if (a.end_out) {
+        a.end_out[4 * r + 0] = x; a.end_out[4 * r + 1] = y;
+        a.end_out[4 * r + 2] = kx; a.end_out[4 * r + 3] = ky;
     }
 }
 
@@ -146,9 +163,9 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a)
     double dx, dy, dkx, dky;
     const BathyView B = load_view<BathyView>(a.B);
     const CurrentView C = load_view<CurrentView>(a.C);
-    BathyCache bc;
-    CurrentCache cc;
-    rhs<BK, CK, MATH>(B, a.bp, bc, C, a.cp, cc, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
+    BathySlot bs;
+    CurrentSlot cs;
+    rhs<BK, CK, MATH>(B, a.bp, bs, C, a.cp, cs, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
                       dx, dy, dkx, dky);
     a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
 }
@@ -237,47 +254,57 @@ __global__ void build_current_kernel(CurrentBuildArgs a)
     }
     const uint32_t t = tiled_cell((int)i, (int)j, a.ntx);
     a.cells[t] = r;
-    if (a.mtab) a.mtab[t] = cell_basis(a.xp, a.yp, i, j);
+    a.mtab[t] = cell_basis(a.xp, a.yp, i, j);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 
-#define MR_DISPATCH(KERNEL, BK, CK, MATH, GRID, BLOCK, STREAM, ARGS)                     \
-    do {                                                                                  \
-        if (MATH == MATH_FAST) {                                                          \
-            DISPATCH_BC(KERNEL, BK, CK, MATH_FAST, GRID, BLOCK, STREAM, ARGS);            \
-        } else {                                                                          \
-            DISPATCH_BC(KERNEL, BK, CK, MATH_LITERAL, GRID, BLOCK, STREAM, ARGS);         \
-        }                                                                                 \
-    } while (0)
+template <int BK, int CK, int MATH>
+struct Kinds {
+    static constexpr int bk = BK, ck = CK, math = MATH;
+};
 
-#define DISPATCH_BC(KERNEL, BK, CK, M, GRID, BLOCK, STREAM, ARGS)                           \
-    do {                                                                                    \
-        if (CK == CK_CONSTANT) {                                                            \
-            switch (BK) {                                                                   \
-            case BK_CONSTANT: hipLaunchKernelGGL((KERNEL<BK_CONSTANT, CK_CONSTANT, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            case BK_SLOPE: hipLaunchKernelGGL((KERNEL<BK_SLOPE, CK_CONSTANT, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            case BK_ARRAY: hipLaunchKernelGGL((KERNEL<BK_ARRAY, CK_CONSTANT, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            default: hipLaunchKernelGGL((KERNEL<BK_GRID, CK_CONSTANT, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            }                                                                               \
-        } else {                                                                            \
-            switch (BK) {                                                                   \
-            case BK_CONSTANT: hipLaunchKernelGGL((KERNEL<BK_CONSTANT, CK_GRID, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            case BK_SLOPE: hipLaunchKernelGGL((KERNEL<BK_SLOPE, CK_GRID, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            case BK_ARRAY: hipLaunchKernelGGL((KERNEL<BK_ARRAY, CK_GRID, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            default: hipLaunchKernelGGL((KERNEL<BK_GRID, CK_GRID, M>), GRID, BLOCK, 0, STREAM, ARGS); break; \
-            }                                                                               \
-        }                                                                                   \
-    } while (0)
+// Call f(Kinds<BK, CK, MATH>{}) for the run-time field kinds and math mode.
+template <class F>
+void dispatch_kinds(int bk, int ck, int math, F &&f)
+{
+    auto by_math = [&](auto b, auto c) {
+        constexpr int BK = decltype(b)::value, CK = decltype(c)::value;
+        if (math == MATH_FAST) f(Kinds<BK, CK, MATH_FAST>{});
+        else f(Kinds<BK, CK, MATH_LITERAL>{});
+    };
+    auto by_current = [&](auto b) {
+        if (ck == CK_CONSTANT) by_math(b, std::integral_constant<int, CK_CONSTANT>{});
+        else by_math(b, std::integral_constant<int, CK_GRID>{});
+    };
+    switch (bk) {
+    case BK_CONSTANT: by_current(std::integral_constant<int, BK_CONSTANT>{}); break;
+    case BK_SLOPE: by_current(std::integral_constant<int, BK_SLOPE>{}); break;
+    case BK_ARRAY: by_current(std::integral_constant<int, BK_ARRAY>{}); break;
+    default: by_current(std::integral_constant<int, BK_GRID>{}); break;
+    }
+}
 
 hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
                         hipStream_t stream)
 {
     if (a.n_rays <= 0) return hipSuccess;
-    const int64_t nblk = (a.n_rays + block - 1) / block;
-    MR_DISPATCH(trace_kernel, bk, ck, math, dim3((unsigned)nblk), dim3(block),
-                stream, a);
+    const dim3 grid((unsigned)((a.n_rays + block - 1) / block));
+    const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
+                     (a.kyo != nullptr);
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using K = decltype(k);
+        if (nout == 0)
+            hipLaunchKernelGGL((trace_kernel<K::bk, K::ck, K::math, 0>), grid, dim3(block), 0,
+                               stream, a);
+        else if (nout == 4)
+            hipLaunchKernelGGL((trace_kernel<K::bk, K::ck, K::math, 1>), grid, dim3(block), 0,
+                               stream, a);
+        else
+            hipLaunchKernelGGL((trace_kernel<K::bk, K::ck, K::math, 2>), grid, dim3(block), 0,
+                               stream, a);
+    });
     return hipGetLastError();
 }
 
@@ -285,9 +312,11 @@ hipError_t launch_rhs(const RhsArgs &a, int bk, int ck, int math, hipStream_t st
 {
     if (a.n <= 0) return hipSuccess;
     const int block = 256;
-    const int64_t nblk = (a.n + block - 1) / block;
-    MR_DISPATCH(rhs_kernel, bk, ck, math, dim3((unsigned)nblk), dim3(block),
-                stream, a);
+    const dim3 grid((unsigned)((a.n + block - 1) / block));
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using K = decltype(k);
+        hipLaunchKernelGGL((rhs_kernel<K::bk, K::ck, K::math>), grid, dim3(block), 0, stream, a);
+    });
     return hipGetLastError();
 }
 
