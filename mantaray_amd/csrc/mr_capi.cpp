@@ -466,8 +466,16 @@ int current_kind_of(const mr_field *c) { return c->kind == MR_CURRENT_CONSTANT ?
 
 int tuned_block(int64_t n_rays)
 {
-    // one wave per workgroup until every SIMD of the device has a wave
-    return n_rays >= 256 * 1024 ? 256 : 64;
+    // spread the rays over every CU (one workgroup each while rays last),
+    // whole waves per workgroup, at most one wave per SIMD per workgroup
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    const int64_t per_cu = (n_rays + cus - 1) / cus;
+    const int64_t waves = std::min<int64_t>(4, std::max<int64_t>(1, (per_cu + 63) / 64));
+    return (int)(64 * waves);
 }
 
 // the trace itself, all pointers device pointers, enqueued on stream

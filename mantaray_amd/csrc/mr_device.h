@@ -454,34 +454,33 @@ __device__ __forceinline__ double rsq_nr(double x)
     return __builtin_fma(0.5 * y, e, y);
 }
 
+// The f64 constants of the fast dispersion evaluation.  Kernels copy them
+// into per-lane registers at start (load_view), so that the loop does not
+// rebuild each one in a scalar register pair at every use.
+struct FastConsts {
+    double inv_ln2, ln2_hi, ln2_lo;   // 1/ln 2; ln 2 split in two parts
+    double clamp;                     // |argument| limit of exp_pair
+    double inv_fact[11];              // 1/13!, 1/12!, ..., 1/3!
+    double g, half_g;                 // g and g/2
+};
+
 // e = exp(-r) and em = 1 - exp(-r), each with small relative error, from one
 // reduced-argument evaluation: r = n ln2 + f, |f| <= ln2/2, expm1(-f) by its
 // Taylor series through degree 13 (truncation below 2^-56 relative on that
 // interval), e = 2^-n (1 + p), em = (1 - 2^-n) - 2^-n p, which has no
 // cancellation for n >= 1 and is exactly -p for n = 0.  |r| is clamped to
 // 1400, beyond which exp(-r) is 0 or overflows in f64 either way.
-__device__ __forceinline__ void exp_pair(double r, double &e, double &em)
+__device__ __forceinline__ void exp_pair(double r, double &e, double &em, const FastConsts &K)
 {
-    const double INV_LN2 = 1.4426950408889634;          // 1/ln 2
-    const double LN2_HI = 6.93147180369123816490e-01;  // ln 2 split, upper part
-    const double LN2_LO = 1.90821492927058770002e-10;  // ln 2 - LN2_HI
-    r = __builtin_fmin(__builtin_fmax(r, -1400.0), 1400.0);
-    const double n = __builtin_rint(r * INV_LN2);
-    double f = __builtin_fma(-n, LN2_HI, r);
-    f = __builtin_fma(-n, LN2_LO, f);
+    r = __builtin_fmin(__builtin_fmax(r, -K.clamp), K.clamp);
+    const double n = __builtin_rint(r * K.inv_ln2);
+    double f = __builtin_fma(-n, K.ln2_hi, r);
+    f = __builtin_fma(-n, K.ln2_lo, f);
     const double g = -f;
     // P(g) = sum_{j=0..12} g^j / (j+1)!,  expm1(g) = g * P(g)
-    double P = 1.6059043836821614e-10;                  // 1/13!
-    P = __builtin_fma(P, g, 2.08767569878680989e-09);   // 1/12!
-    P = __builtin_fma(P, g, 2.50521083854417188e-08);   // 1/11!
-    P = __builtin_fma(P, g, 2.75573192239858907e-07);   // 1/10!
-    P = __builtin_fma(P, g, 2.75573192239858907e-06);   // 1/9!
-    P = __builtin_fma(P, g, 2.48015873015873016e-05);   // 1/8!
-    P = __builtin_fma(P, g, 1.98412698412698413e-04);   // 1/7!
-    P = __builtin_fma(P, g, 1.38888888888888889e-03);   // 1/6!
-    P = __builtin_fma(P, g, 8.33333333333333333e-03);   // 1/5!
-    P = __builtin_fma(P, g, 4.16666666666666667e-02);   // 1/4!
-    P = __builtin_fma(P, g, 1.66666666666666667e-01);   // 1/3!
+    double P = K.inv_fact[0];
+#pragma unroll
+    for (int j = 1; j < 11; ++j) P = __builtin_fma(P, g, K.inv_fact[j]);
     P = __builtin_fma(P, g, 0.5);                       // 1/2!
     P = __builtin_fma(P, g, 1.0);                       // 1/1!
     const double p = g * P;
@@ -496,8 +495,9 @@ __device__ __forceinline__ void exp_pair(double r, double &e, double &em)
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
                                     const CurrentView &C, const CurrentPtrs &CP,
-                                    CurrentSlot &cs, double x, double y, double kx, double ky,
-                                    double &dx, double &dy, double &dkx, double &dky)
+                                    CurrentSlot &cs, const FastConsts &K, double x, double y,
+                                    double kx, double ky, double &dx, double &dy, double &dkx,
+                                    double &dky)
 {
     double hx, hy, u, v, ux, uy, vx, vy;
     double h = bathy_query<BK>(B, BP, bs, x, y, hx, hy);             // odes :120-127
@@ -542,16 +542,16 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
         const double kh = k * h;
         const double cth = kx * rk, sth = ky * rk;          // cos, sin of atan2(ky, kx)
         double e, em;
-        exp_pair(2.0 * kh, e, em);                          // e = exp(-2kh)
+        exp_pair(2.0 * kh, e, em, K);                       // e = exp(-2kh)
         const double ope = 1.0 + e;
         const double D = rcp_nr(em * ope);                  // 1/(em (1+e))
         const double iope = em * D;                         // 1/(1+e)
         const double T = em * iope;                         // tanh(kh)
         const double sech2 = (4.0 * e) * (iope * iope);     // 1/cosh^2(kh)
-        const double gkT = (G * k) * T;
+        const double gkT = (K.g * k) * T;
         const double rs = rsq_nr(gkT);
         const double sigma = gkT * rs;                      // sqrt(g k tanh kh)
-        const double cg = (0.5 * G) * __builtin_fma(kh, sech2, T) * rs;
+        const double cg = K.half_g * __builtin_fma(kh, sech2, T) * rs;
         // (-k/2)/(sinh cosh) * sigma = -2 k e sigma / (em (1+e))
         const double c = (-2.0 * k) * e * sigma * D;
         dx = __builtin_fma(cg, cth, u);

@@ -35,6 +35,26 @@ __device__ __forceinline__ View load_view(const void *p)
     return v;
 }
 
+__device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
+    1.4426950408889634,                    // 1/ln 2
+    6.93147180369123816490e-01,            // ln 2 split, upper part
+    1.90821492927058770002e-10,            // ln 2 - upper part
+    1400.0,
+    1.6059043836821614e-10,                // 1/13!
+    2.08767569878680989e-09,               // 1/12!
+    2.50521083854417188e-08,               // 1/11!
+    2.75573192239858907e-07,               // 1/10!
+    2.75573192239858907e-06,               // 1/9!
+    2.48015873015873016e-05,               // 1/8!
+    1.98412698412698413e-04,               // 1/7!
+    1.38888888888888889e-03,               // 1/6!
+    8.33333333333333333e-03,               // 1/5!
+    4.16666666666666667e-02,               // 1/4!
+    1.66666666666666667e-01,               // 1/3!
+    G,
+    0.5 * G,
+};
+
 // ---------------------------------------------------------------------------
 // ode_solvers 0.4.0 Rk4::step/integrate for one lane:
 //   k1 = f(y), k2 = f(y + k1 (dt/2)), k3 = f(y + k2 (dt/2)), k4 = f(y + k3 dt)
@@ -70,6 +90,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     }
     const BathyView B = load_view<BathyView>(a.B);
     const CurrentView C = load_view<CurrentView>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     const double half = a.half, dt = a.dt, sixth = a.sixth;
     BathySlot bn;                      // records prefetched for the next evaluation
     CurrentSlot cn;
@@ -100,7 +121,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                 if (PF_C && !current_inside(cq, gx, gy))
                     current_prefetch(C, a.cp, cn, gx, gy);
                 double dx, dy, dkx, dky;
-                rhs<BK, CK, MATH>(B, a.bp, bq, C, a.cp, cq, sx, sy, skx, sky, dx, dy, dkx, dky);
+                rhs<BK, CK, MATH>(B, a.bp, bq, C, a.cp, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
                 px = dx; py = dy;
                 if (st == 0) {
                     // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
@@ -163,9 +184,10 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a)
     double dx, dy, dkx, dky;
     const BathyView B = load_view<BathyView>(a.B);
     const CurrentView C = load_view<CurrentView>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     BathySlot bs;
     CurrentSlot cs;
-    rhs<BK, CK, MATH>(B, a.bp, bs, C, a.cp, cs, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
+    rhs<BK, CK, MATH>(B, a.bp, bs, C, a.cp, cs, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
                       dx, dy, dkx, dky);
     a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
 }
