@@ -152,7 +152,13 @@ def single_ray(x0: float, y0: float, kx0: float, ky0: float, duration: float,
     row including the terminal all-NaN one; time, x, y are coordinates.
     """
     b, c = _fields(bathymetry, current)
-    r = trace_many(b, c, [x0], [y0], [kx0], [ky0], 0.0, duration, step_size, **kw)
+    return single_dataset(trace_many(b, c, [x0], [y0], [kx0], [ky0], 0.0, duration, step_size,
+                                     **kw))
+
+
+def single_dataset(r: dict):
+    """The single_ray Dataset (core.py:56-63) of a one-ray trace_many result:
+    every row of the ray, the terminal all-NaN one included."""
     ln = int(r["len"][0])
     cols = [r["t"][:ln], r["x"][:ln, 0], r["y"][:ln, 0], r["kx"][:ln, 0], r["ky"][:ln, 0]]
     varnames = ["time", "x", "y", "kx", "ky"]
@@ -172,7 +178,12 @@ def ray_tracing(x0: Sequence[float], y0: Sequence[float], kx0: Sequence[float],
     ray, as core.py:114-125 pads the per-ray lists.
     """
     b, c = _fields(bathymetry, current)
-    r = trace_many(b, c, x0, y0, kx0, ky0, 0.0, duration, step_size, **kw)
+    return rays_dataset(trace_many(b, c, x0, y0, kx0, ky0, 0.0, duration, step_size, **kw))
+
+
+def rays_dataset(r: dict):
+    """The ray_tracing Dataset (core.py:104-132) of a trace_many result: each
+    ray's rows, then all-NaN rows (time too) up to the longest ray."""
     ln = r["len"]
     nr = ln.size
     if nr == 0:

@@ -96,6 +96,13 @@ def lib():
                                      ctypes.c_int, ctypes.c_int64, _f64p, _f64p, _i32p]
         L.or_system_many.restype = None
         L.or_max_threads.restype = ctypes.c_int
+        L.or_bathy_nearest.argtypes = [ctypes.POINTER(_Bathy), ctypes.c_float, ctypes.c_int, _f32p]
+        L.or_current_nearest.argtypes = [ctypes.POINTER(_Current), ctypes.c_double, ctypes.c_int,
+                                         _f64p]
+        L.or_bathy_corners.argtypes = [ctypes.POINTER(_Bathy), ctypes.c_float, ctypes.c_float,
+                                       ctypes.POINTER(ctypes.c_uint64 * 8)]
+        L.or_current_corners.argtypes = [ctypes.POINTER(_Current), ctypes.c_double,
+                                         ctypes.c_double, ctypes.POINTER(ctypes.c_uint64 * 8)]
         _lib = L
     return _lib
 
@@ -187,6 +194,30 @@ def bilinear32(points, target):
     out = ctypes.c_float()
     st = lib().or_bilinear32(ctypes.byref(arr), target[0], target[1], ctypes.byref(out))
     return st, out.value
+
+
+def bathy_nearest(b: Bathymetry, target, axis):
+    idx = ctypes.c_float()
+    st = lib().or_bathy_nearest(ctypes.byref(b.s), np.float32(target), axis, ctypes.byref(idx))
+    return st, np.float32(idx.value)
+
+
+def current_nearest(c: Current, target, axis):
+    idx = ctypes.c_double()
+    st = lib().or_current_nearest(ctypes.byref(c.s), target, axis, ctypes.byref(idx))
+    return st, idx.value
+
+
+def bathy_corners(b: Bathymetry, px, py):
+    out = (ctypes.c_uint64 * 8)()
+    st = lib().or_bathy_corners(ctypes.byref(b.s), np.float32(px), np.float32(py), out)
+    return st, [(int(out[2 * q]), int(out[2 * q + 1])) for q in range(4)]
+
+
+def current_corners(c: Current, x, y):
+    out = (ctypes.c_uint64 * 8)()
+    st = lib().or_current_corners(ctypes.byref(c.s), x, y, out)
+    return st, [(int(out[2 * q]), int(out[2 * q + 1])) for q in range(4)]
 
 
 def bathy_query(b: Bathymetry, px, py):

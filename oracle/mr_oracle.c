@@ -183,6 +183,46 @@ static void corners_of(uint64_t x1, uint64_t x2, uint64_t y1, uint64_t y2, uint6
     ci[3][0] = x2; ci[3][1] = y1;
 }
 
+/* cartesian_netcdf3.rs:274-296 / cartesian_current.rs:231-253 -- nearest
+ * along axis 0 (x) or 1 (y) of a gridded field */
+int or_bathy_nearest(const or_bathy *b, float target, int axis, float *idx)
+{
+    return axis == 0 ? nearest32(target, b->x, b->nx, idx) : nearest32(target, b->y, b->ny, idx);
+}
+
+int or_current_nearest(const or_current *c, double target, int axis, double *idx)
+{
+    return axis == 0 ? nearest64(target, c->x, c->nx, idx) : nearest64(target, c->y, c->ny, idx);
+}
+
+/* cartesian_netcdf3.rs:334-390 / cartesian_current.rs:283-339 -- four_corners:
+ * [(x1,y1), (x1,y2), (x2,y2), (x2,y1)] as 8 integers */
+int or_bathy_corners(const or_bathy *b, float px, float py, uint64_t out[8])
+{
+    float xi, yi;
+    if (nearest32(px, b->x, b->nx, &xi) != OR_OK) return OR_ERR;
+    if (nearest32(py, b->y, b->ny, &yi) != OR_OK) return OR_ERR;
+    uint64_t x1, x2, y1, y2, ci[4][2];
+    cell_axis32(xi, b->nx, &x1, &x2);
+    cell_axis32(yi, b->ny, &y1, &y2);
+    corners_of(x1, x2, y1, y2, ci);
+    for (int q = 0; q < 4; q++) { out[2 * q] = ci[q][0]; out[2 * q + 1] = ci[q][1]; }
+    return OR_OK;
+}
+
+int or_current_corners(const or_current *c, double x, double y, uint64_t out[8])
+{
+    double xi, yi;
+    if (nearest64(x, c->x, c->nx, &xi) != OR_OK) return OR_ERR;
+    if (nearest64(y, c->y, c->ny, &yi) != OR_OK) return OR_ERR;
+    uint64_t x1, x2, y1, y2, ci[4][2];
+    cell_axis64(xi, c->nx, &x1, &x2);
+    cell_axis64(yi, c->ny, &y1, &y2);
+    corners_of(x1, x2, y1, y2, ci);
+    for (int q = 0; q < 4; q++) { out[2 * q] = ci[q][0]; out[2 * q + 1] = ci[q][1]; }
+    return OR_OK;
+}
+
 /* cartesian_netcdf3.rs:98-135 (grid) plus the analytic providers
  * constant_depth.rs:26-33, constant_slope.rs:63-74, array_depth.rs:26-33 */
 int or_bathy_depth_and_gradient(const or_bathy *b, float px, float py,

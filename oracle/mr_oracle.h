@@ -54,6 +54,10 @@ typedef struct {
 } or_current;
 
 int or_bilinear32(const float pts[4][3], float tx, float ty, float *out);
+int or_bathy_nearest(const or_bathy *b, float target, int axis, float *idx);
+int or_current_nearest(const or_current *c, double target, int axis, double *idx);
+int or_bathy_corners(const or_bathy *b, float px, float py, uint64_t out[8]);
+int or_current_corners(const or_current *c, double x, double y, uint64_t out[8]);
 int or_bathy_depth_and_gradient(const or_bathy *b, float px, float py,
                                 float *h, float *hx, float *hy);
 int or_current_and_gradient(const or_current *c, double x, double y,

@@ -1,0 +1,145 @@
+"""The C ABI of libmantaray_hip.so on the CPU: the library loads, exports
+every function include/mantaray_capi.h declares, and its host-side logic
+(argument checks, step count, field handles, error reporting) behaves as
+documented.  No call here launches device work."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mantaray_amd import _capi as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mantaray_capi.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(mr_[a-z0-9_]+)\s*\(", text))
+
+
+def test_header_matches_binding():
+    assert header_functions() == set(C.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    L = C.lib()
+    for name in header_functions():
+        assert hasattr(L, name), name
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(C, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(C, "_lib", None)
+    with pytest.raises(ImportError):
+        C.lib()
+
+
+def test_version_and_options():
+    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert C.lib().mr_version(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+    assert (a.value, b.value, c.value) >= (0, 1, 0)
+    o = C.options()
+    assert o.struct_size == ctypes.sizeof(C.Options)
+    assert (o.precision, o.math, o.solout, o.memory, o.device, o.block_size) == \
+        (C.MR_PREC_REFERENCE, C.MR_MATH_FAST, C.MR_SOLOUT_STATE, C.MR_MEM_HOST, -1, 0)
+
+
+def _steps(t0, t1, dt):
+    n = ctypes.c_int64()
+    rc = C.lib().mr_num_steps(t0, t1, dt, ctypes.byref(n))
+    return rc, n.value
+
+
+@pytest.mark.parametrize("t0,t1,dt,n", [(0.0, 10.0, 2.0, 5), (0.0, 10.0, 3.0, 4),
+                                        (0.0, 0.0, 1.0, 0), (1.0, 10.0, 1.0, 9),
+                                        (100.0, 102.0, 1.0, 2), (0.0, 6.8, 0.1, 68),
+                                        (0.0, -0.5, 1.0, 0), (0.0, -0.5, 2.0, 0)])
+def test_num_steps(oracle_mod, t0, t1, dt, n):
+    """Rk4::integrate: ceil((t_end - t0)/dt) converted by num-traits, which
+    accepts -1 < q < 2^64 (so a ceil of -0.x is 0 steps)."""
+    assert _steps(t0, t1, dt) == (0, n)
+    assert oracle_mod.num_steps(t0, t1, dt) == n
+
+
+@pytest.mark.parametrize("t0,t1,dt", [(0.0, -1.0, 1.0), (0.0, 10.0, -1.0), (0.0, 10.0, 0.0),
+                                      (0.0, math.nan, 1.0), (0.0, 10.0, math.nan),
+                                      (0.0, math.inf, 1.0), (0.0, 1e300, 1e-300)])
+def test_num_steps_panics(oracle_mod, t0, t1, dt):
+    """Where to_usize().unwrap() panics the ABI returns MR_ERR_STEP_COUNT."""
+    assert _steps(t0, t1, dt)[0] == C.MR_ERR_STEP_COUNT
+    assert oracle_mod.num_steps(t0, t1, dt) < 0 or (t0, t1, dt) == (0.0, 1e300, 1e-300)
+
+
+def test_field_handles_round_trip():
+    import mantaray_amd as mr
+    x = np.array([0.0, 1.5, 3.0], dtype=np.float32)
+    y = np.array([-2.0, 0.0], dtype=np.float32)
+    d = np.arange(6.0)
+    b = mr.Bathymetry.from_arrays(x, y, d)
+    assert b.kind == C.MR_BATHY_GRID
+    gx, gy, gd = b.grid()
+    assert np.array_equal(gx, x.astype(np.float64)) and np.array_equal(gy, y)
+    assert np.array_equal(gd, d)
+    c = mr.Current.from_arrays(x.astype(np.float64), y.astype(np.float64), d, -d)
+    assert c.kind == C.MR_CURRENT_GRID
+    cx, cy, cu, cv = c.grid()
+    assert np.array_equal(cu, d) and np.array_equal(cv, -d)
+    assert mr.Bathymetry.constant(5.0).kind == C.MR_BATHY_CONSTANT
+    assert mr.Bathymetry.slope().kind == C.MR_BATHY_SLOPE
+    assert mr.Bathymetry.array(np.ones((3, 3))).kind == C.MR_BATHY_ARRAY
+    assert mr.Current.constant(1.0, 2.0).kind == C.MR_CURRENT_CONSTANT
+    with pytest.raises(ValueError):
+        mr.Bathymetry.array(np.ones((2, 3)))
+
+
+def test_netcdf_errors(tmp_path):
+    import mantaray_amd as mr
+    with pytest.raises(C.MantarayError) as e:
+        mr.Bathymetry.from_netcdf(tmp_path / "missing.nc")
+    assert e.value.code == C.MR_ERR_IO
+    (tmp_path / "junk.nc").write_bytes(b"not a netcdf file at all")
+    with pytest.raises(C.MantarayError) as e:
+        mr.Bathymetry.from_netcdf(tmp_path / "junk.nc")
+    assert e.value.code == C.MR_ERR_FORMAT
+    from _ncfix import create_bathymetry
+    create_bathymetry(tmp_path / "b.nc", 4, 3, 1.0, 1.0, lambda x, y: x + y)
+    with pytest.raises(C.MantarayError) as e:
+        mr.Bathymetry.from_netcdf(tmp_path / "b.nc", depth_name="elevation")
+    assert e.value.code == C.MR_ERR_FORMAT
+    assert "elevation" in str(e.value)
+
+
+def _trace_rc(b, c, n, opts=None, ic=True):
+    z = np.zeros(max(n, 1))
+    p = z.ctypes.data if ic else None
+    L = C.lib()
+    return L.mr_trace_many(b, c, n, p, p, p, p, 0.0, 1.0, 1.0,
+                           ctypes.byref(opts or C.options()), None, None, None, None, None,
+                           None, None)
+
+
+def test_trace_argument_errors():
+    """Rejected before any device work: wrong roles, null fields, negative
+    counts, null initial conditions, bad options, invalid step counts."""
+    import mantaray_amd as mr
+    b, c = mr.Bathymetry.constant(10.0), mr.Current.constant()
+    L = C.lib()
+    assert _trace_rc(c.handle, b.handle, 1) == C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(None, c.handle, 1) == C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(b.handle, c.handle, -1) == C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(b.handle, c.handle, 1, ic=False) == C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(b.handle, c.handle, 1, C.options(block_size=100)) == \
+        C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(b.handle, c.handle, 1, C.options(math=7)) == C.MR_ERR_INVALID_ARGUMENT
+    assert _trace_rc(b.handle, c.handle, 1, C.options(struct_size=4)) == \
+        C.MR_ERR_INVALID_ARGUMENT
+    z = np.zeros(1)
+    rc = L.mr_trace_many(b.handle, c.handle, 1, *([z.ctypes.data] * 4), 0.0, -5.0, 1.0,
+                         ctypes.byref(C.options()), *([None] * 7))
+    assert rc == C.MR_ERR_STEP_COUNT
+    assert L.mr_last_error()
