@@ -11,8 +11,9 @@ K back-to-back launches of the trace kernel.
 
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
-its own event-timed duration), and the CPU oracle timed on a bounded sample
-of the same workload on this host's cores.
+its own event-timed duration; HBM bytes per launch from the rocprofv3
+counter summary in profiles/), and the CPU oracle (OpenMP over rays, all
+host cores) timed on the same workload, all 65,536 rays by default.
 """
 from __future__ import annotations
 
@@ -28,6 +29,8 @@ if ROOT not in sys.path:
 
 FLOP_PER_RAY_STEP = 745          # 265 basic FP64 ops + 24 transcendentals x 20
 PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 (vector and matrix)
+# HBM bytes per launch of the trace kernel on this workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_kernel_seamount_jet_traffic.json")
 
 
@@ -41,7 +44,8 @@ def parse():
     p.add_argument("--math", default="fast", choices=["fast", "literal"])
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-rays", type=int, default=16384)
+    p.add_argument("--cpu-rays", type=int, default=65536,
+                   help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
 
 
@@ -139,11 +143,13 @@ def main():
                    "rk4_steps": n, "dt_s": cfg["dt"], "ray_steps_per_launch": ray_steps,
                    "outputs": "full step-major trajectories (x, y, kx, ky) in HBM",
                    "math": args.math, "parallelism": "rays, one per lane"},
-        "roofline": {"bound": "mfma", "roof": "FP64 vector/matrix peak",
+        "roofline": {"bound": "mfma", "roof": "FP64 (vector and matrix share the peak)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel_ms": kern_ms,
-                     "work": f"{FLOP_PER_RAY_STEP} flop/ray-step x {ray_steps} ray-steps"},
+                     "work": f"{FLOP_PER_RAY_STEP} flop/ray-step x {ray_steps} ray-steps",
+                     "hbm_GBps": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9,
+                     "hbm_peak_GBps": 8000.0},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_rays, nr))

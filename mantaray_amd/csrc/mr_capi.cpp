@@ -489,7 +489,7 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     int rc = field_tables(bathy, &bt);
     if (!rc) rc = field_tables(current, &ct);
     if (rc) return rc;
-    if (t_out) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
+    if (t_out && n_rays == 0) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
     TraceArgs a;
     a.B = bt->view;
     a.C = ct->view;
@@ -498,6 +498,7 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.n_rays = n_rays;
     a.n_steps = n;
     a.t0 = t0;
+    a.t_out = t_out;
     a.dt = dt;
     a.half = dt / 2.0;                                 // Rk4::new half_step
     a.sixth = dt / 6.0;                                // Rk4::step (step_size / 6)
@@ -507,6 +508,7 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.end_out = end_out;
     a.solout = o.solout;
     const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
                             stream));
     return MR_OK;

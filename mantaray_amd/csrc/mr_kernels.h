@@ -15,6 +15,8 @@ struct TraceArgs {
     CurrentPtrs cp;
     int64_t n_rays, n_steps;
     double t0, dt, half, sixth;
+    double *t_out;                     // n_steps + 1 times, may be null
+    int ray_blocks;                    // workgroups of rays; one more writes t_out
     const double *x0, *y0, *kx0, *ky0;
     double *xo, *yo, *kxo, *kyo;       // step-major, may be null
     int32_t *len_out;                  // may be null

@@ -8,7 +8,9 @@
 //                  batch of states (parity tests of the right-hand side).
 //   build_*        per-cell records of a gridded field (see mr_device.h).
 //   fill_time      the time column, t += dt accumulated as ode_solvers does
-//                  (one lane: the accumulation is a sequential rounding chain).
+//                  (one lane: the accumulation is a sequential rounding chain),
+//                  for a trace with no rays; otherwise the trace kernel's last
+//                  workgroup writes it beside the rays.
 #include "mr_kernels.h"
 
 #include <type_traits>
@@ -55,6 +57,17 @@ __device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
     0.5 * G,
 };
 
+// t[0] = t0, t[s] = t[s-1] + dt for s <= n
+__device__ __forceinline__ void time_column(double t0, double dt, int64_t n, double *t)
+{
+    double v = t0;
+    t[0] = v;
+    for (int64_t s = 1; s <= n; ++s) {
+        v = v + dt;
+        t[s] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // ode_solvers 0.4.0 Rk4::step/integrate for one lane:
 //   k1 = f(y), k2 = f(y + k1 (dt/2)), k3 = f(y + k2 (dt/2)), k4 = f(y + k3 dt)
@@ -76,6 +89,12 @@ template <int BK, int CK, int MATH, int OUT>
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
+    if ((int)blockIdx.x == a.ray_blocks) {
+        // the time column: t_s = t_{s-1} + dt (ode_solvers accumulates), a
+        // sequential rounding chain, run by one lane beside the rays
+        if (threadIdx.x == 0) time_column(a.t0, a.dt, a.n_steps, a.t_out);
+        return;
+    }
     const int64_t nr = a.n_rays;
     const int64_t r = __builtin_elementwise_min((int64_t)blockIdx.x * blockDim.x + threadIdx.x,
                                                 nr - 1);
@@ -194,14 +213,7 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a)
 
 __global__ void fill_time_kernel(double t0, double dt, int64_t n, double *t)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double v = t0;
-        t[0] = v;
-        for (int64_t s = 1; s <= n; ++s) {
-            v = v + dt;
-            t[s] = v;
-        }
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) time_column(t0, dt, n, t);
 }
 
 // Cell records (mr_device.h).  Cell (i, j) has corners A = (i, j),
@@ -312,7 +324,8 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
                         hipStream_t stream)
 {
     if (a.n_rays <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((a.n_rays + block - 1) / block));
+    if (a.ray_blocks != (int)((a.n_rays + block - 1) / block)) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
     const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
                      (a.kyo != nullptr);
     dispatch_kinds(bk, ck, math, [&](auto k) {
