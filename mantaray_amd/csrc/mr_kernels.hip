@@ -170,7 +170,12 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         }
         const int64_t o = s * nr + r;
         if (OUT == 1) {
-            a.xo[o] = ox; a.yo[o] = oy; a.kxo[o] = okx; a.kyo[o] = oky;
+            // streaming stores: the rows are not read again by the kernel,
+            // and keeping them out of L2 leaves it to the cell records
+            __builtin_nontemporal_store(ox, a.xo + o);
+            __builtin_nontemporal_store(oy, a.yo + o);
+            __builtin_nontemporal_store(okx, a.kxo + o);
+            __builtin_nontemporal_store(oky, a.kyo + o);
         } else if (OUT == 2) {
             if (a.xo) a.xo[o] = ox;
             if (a.yo) a.yo[o] = oy;
@@ -182,10 +187,10 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     if (OUT != 0) {
         for (; s <= a.n_steps; ++s) {
             const int64_t o = s * nr + r;
-            if (OUT == 1 || a.xo) a.xo[o] = MR_NAN;
-            if (OUT == 1 || a.yo) a.yo[o] = MR_NAN;
-            if (OUT == 1 || a.kxo) a.kxo[o] = MR_NAN;
-            if (OUT == 1 || a.kyo) a.kyo[o] = MR_NAN;
+            if (OUT == 1 || a.xo) __builtin_nontemporal_store(MR_NAN, a.xo + o);
+            if (OUT == 1 || a.yo) __builtin_nontemporal_store(MR_NAN, a.yo + o);
+            if (OUT == 1 || a.kxo) __builtin_nontemporal_store(MR_NAN, a.kxo + o);
+            if (OUT == 1 || a.kyo) __builtin_nontemporal_store(MR_NAN, a.kyo + o);
         }
     }
     if (a.len_out) a.len_out[r] = (int32_t)len;
