@@ -13,7 +13,8 @@ Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
 its own event-timed duration; HBM bytes per launch from the rocprofv3
 counter summary in profiles/), and the CPU oracle (OpenMP over rays, all
-host cores) timed on the same workload, all 65,536 rays by default.
+host cores) timed on the first 16,384 rays of the same workload (median of
+three runs), with a single-thread figure beside it.
 """
 from __future__ import annotations
 
@@ -44,29 +45,56 @@ def parse():
     p.add_argument("--math", default="fast", choices=["fast", "literal"])
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-rays", type=int, default=65536,
+    p.add_argument("--cpu-rays", type=int, default=16384,
                    help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
 
 
-def cpu_baseline(cfg, n_rays):
-    """The CPU oracle (OpenMP over rays) on the first n_rays rays."""
+def _host():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(cfg, n_rays, runs=3, single_rays=256):
+    """The CPU oracle (OpenMP over rays, dynamic schedule, all host threads)
+    on the first n_rays rays: median of `runs` timed runs after a warm-up,
+    plus a single-thread figure on the first single_rays rays."""
     import numpy as np
     import oracle
     oracle.build()
     b = oracle.Bathymetry.grid(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
     c = oracle.Current.grid(cfg["current"]["x"], cfg["current"]["y"], cfg["current"]["u"],
                             cfg["current"]["v"])
-    sl = slice(0, n_rays)
-    t0 = time.perf_counter()
-    r = oracle.trace_many(b, c, cfg["x0"][sl], cfg["y0"][sl], cfg["kx0"][sl], cfg["ky0"][sl],
-                          cfg["t0"], cfg["t_end"], cfg["dt"], trajectories=False)
-    el = time.perf_counter() - t0
-    steps = int((r["len"].astype(np.int64) - 1).sum())
+
+    def run(n, threads):
+        sl = slice(0, n)
+        t0 = time.perf_counter()
+        r = oracle.trace_many(b, c, cfg["x0"][sl], cfg["y0"][sl], cfg["kx0"][sl], cfg["ky0"][sl],
+                              cfg["t0"], cfg["t_end"], cfg["dt"], trajectories=False,
+                              nthreads=threads)
+        el = time.perf_counter() - t0
+        return int((r["len"].astype(np.int64) - 1).sum()), el
+
+    run(min(n_rays, 1024), 0)                                   # warm-up
+    timed = [run(n_rays, 0) for _ in range(runs)]
+    steps = timed[0][0]
+    el = float(np.median([t for _, t in timed]))
+    s1, e1 = run(single_rays, 1)
+    model, ncpu = _host()
     return {"value": steps / el, "unit": "ray-steps/s", "cores": oracle.max_threads(),
             "kind": "port",
-            "sample": f"first {n_rays} rays of the same workload, all steps "
-                      f"({steps} ray-steps in {el:.2f} s)"}
+            "sample": f"first {n_rays} rays of the same workload, all steps ({steps} ray-steps), "
+                      f"median of {runs} runs ({el:.2f} s each)",
+            "single_thread": {"value": s1 / e1, "sample": f"first {single_rays} rays, 1 thread"},
+            "host": {"cpu": model, "nproc": ncpu}}
 
 
 def main():
