@@ -59,10 +59,10 @@ extern "C" {
 
 /* mr_options.precision */
 #define MR_PREC_REFERENCE   0   /* the reference's f32 interpolation, f64 state */
-#define MR_PREC_FP64_INTERP 1   /* all interpolation in f64 (differentiable) */
+#define MR_PREC_FP64_INTERP 1   /* all interpolation in f64 (differentiable; see below) */
 
 /* mr_options.math */
-#define MR_MATH_FAST    0       /* dispersion terms from one exp; see DESIGN */
+#define MR_MATH_FAST    0       /* dispersion terms in closed form from one exp */
 #define MR_MATH_LITERAL 1       /* atan2/cos/sin/tanh/cosh/sinh as written */
 
 /* mr_options.solout: what stops a ray (src/wave_ray_path.rs:236-246) */
@@ -175,6 +175,64 @@ int mr_single_ray(const mr_field *bathy, const mr_field *current,
  * (s[4i..4i+3] = x, y, kx, ky): ds = derivatives, all-NaN where odes errs. */
 int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n,
                 const double *s, double *ds, const mr_options *opt);
+
+/* ---- differentiable path: fp64 interpolation and the adjoint ----------
+ *
+ * No counterpart in the reference, which has no derivatives.  These run the
+ * reference's algorithm (ManyRays::trace_many, src/ray.rs:98-127, and the
+ * providers of src/bathymetry/cartesian_netcdf3.rs and
+ * src/current/cartesian_current.rs) with every interpolation in f64
+ * (MR_PREC_FP64_INTERP; the bathymetry is queried at the f64 point), and the
+ * discrete adjoint of that computation.  Both take device memory only
+ * (mr_options.memory == MR_MEM_DEVICE) and enqueue on mr_options.stream.
+ *
+ * depth, u, v: device arrays that replace the fields' own values (same
+ * lengths as the fields' arrays: depth_len, u_len, v_len), or NULL to use
+ * them.  They let a caller differentiate with respect to the values and
+ * change them between calls without building new fields.
+ *
+ * Checkpoints: with stride K, checkpoint c (0 <= c < mr_checkpoint_count)
+ * is the state at row c*K, stored step-major SoA: ckpt[(4c + q) * n_rays + r]
+ * for q = x, y, kx, ky. */
+
+/* Number of checkpoints of a trace: max(1, ceil(n / stride)). */
+int mr_checkpoint_count(double t0, double t_end, double dt, int32_t stride, int64_t *count);
+
+/* Bytes of the work buffer mr_adjoint_many needs: stride * 16 * n_rays doubles. */
+int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes);
+
+/* mr_trace_many in fp64-interpolation mode (whatever mr_options.precision
+ * says), with value overrides and, when ckpt is not NULL, checkpoints every
+ * `stride` steps.  Outputs as mr_trace_many; each may be NULL. */
+int mr_trace_fp64(const mr_field *bathy, const mr_field *current,
+                  const double *depth, const double *u, const double *v,
+                  int64_t n_rays, const double *x0, const double *y0,
+                  const double *kx0, const double *ky0,
+                  double t0, double t_end, double dt, int32_t stride,
+                  const mr_options *opt, double *ckpt,
+                  double *t_out, double *x_out, double *y_out, double *kx_out,
+                  double *ky_out, int32_t *len_out, double *end_out);
+
+/* Reverse sweep of the discrete adjoint of mr_trace_fp64 (MR_SOLOUT_STATE).
+ * Inputs: the checkpoints, len_out and end_out of the forward call (same
+ * fields, values, t0, t_end, dt, stride and math mode), and the cotangents
+ * of a loss: g_end [4*n_rays] on each ray's last row (end_out's layout),
+ * and/or g_x .. g_ky [(n+1)*n_rays] on the rows (step-major); any may be
+ * NULL.  A ray's all-NaN last row (edge exit) receives no cotangent.
+ * Outputs (each may be NULL): the gradients with respect to the initial
+ * states g_x0 .. g_ky0 [n_rays] and to the gridded values g_depth
+ * [depth_len], g_u [u_len], g_v [v_len]; the grids are cleared first and
+ * summed with f64 atomics, so they vary in the last bits between runs.
+ * work: device buffer of mr_adjoint_workspace_size bytes. */
+int mr_adjoint_many(const mr_field *bathy, const mr_field *current,
+                    const double *depth, const double *u, const double *v,
+                    int64_t n_rays, double t0, double t_end, double dt, int32_t stride,
+                    const mr_options *opt, const double *ckpt, const int32_t *len,
+                    const double *end, const double *g_end,
+                    const double *g_x, const double *g_y, const double *g_kx, const double *g_ky,
+                    void *work, int64_t work_bytes,
+                    double *g_x0, double *g_y0, double *g_kx0, double *g_ky0,
+                    double *g_depth, double *g_u, double *g_v);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,8 @@ EXPORTS = (
     "mr_current_from_arrays", "mr_bathy_constant", "mr_bathy_slope", "mr_bathy_array",
     "mr_current_constant", "mr_field_free", "mr_field_kind", "mr_field_grid_shape",
     "mr_field_copy_grid", "mr_field_upload", "mr_num_steps", "mr_trace_many",
-    "mr_single_ray", "mr_rhs_many",
+    "mr_single_ray", "mr_rhs_many", "mr_checkpoint_count", "mr_adjoint_workspace_size",
+    "mr_trace_fp64", "mr_adjoint_many",
 )
 
 
@@ -99,6 +100,15 @@ _SIGS = {
     "mr_single_ray": (ctypes.c_int, [_fieldp, _fieldp, _f64, _f64, _f64, _f64, _f64, _f64,
                                      _f64, ctypes.POINTER(Options), _vp, _i64p]),
     "mr_rhs_many": (ctypes.c_int, [_fieldp, _fieldp, _i64, _vp, _vp, ctypes.POINTER(Options)]),
+    "mr_checkpoint_count": (ctypes.c_int, [_f64, _f64, _f64, ctypes.c_int32, _i64p]),
+    "mr_adjoint_workspace_size": (ctypes.c_int, [_i64, ctypes.c_int32, _i64p]),
+    "mr_trace_fp64": (ctypes.c_int, [_fieldp, _fieldp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                                     _f64, _f64, _f64, ctypes.c_int32, ctypes.POINTER(Options),
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mr_adjoint_many": (ctypes.c_int, [_fieldp, _fieldp, _vp, _vp, _vp, _i64, _f64, _f64, _f64,
+                                       ctypes.c_int32, ctypes.POINTER(Options), _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                                       _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -29,6 +29,7 @@ from . import _dataset
 
 _MATH = {"fast": C.MR_MATH_FAST, "literal": C.MR_MATH_LITERAL}
 _SOLOUT = {"state": C.MR_SOLOUT_STATE, "next_deriv": C.MR_SOLOUT_NEXT_DERIV}
+_PREC = {"reference": C.MR_PREC_REFERENCE, "fp64": C.MR_PREC_FP64_INTERP}
 
 
 def _Dataset(data_vars, attrs=None):
@@ -47,15 +48,18 @@ def num_steps(t0: float, t_end: float, dt: float) -> int:
     return n.value
 
 
-def _opts(math="fast", solout="state", **kw):
-    return C.options(math=_MATH[math], solout=_SOLOUT[solout], **kw)
+def _opts(math="fast", solout="state", precision="reference", **kw):
+    return C.options(math=_MATH[math], solout=_SOLOUT[solout], precision=_PREC[precision], **kw)
 
 
 def trace_many(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
                t0: float = 0.0, t_end: float = 1.0, dt: float = 1.0, *,
                math: str = "fast", solout: str = "state", trajectories: bool = True,
-               block_size: int = 0, device: int = -1) -> dict:
+               block_size: int = 0, device: int = -1, precision: str = "reference") -> dict:
     """Trace rays from host arrays; returns a dict of numpy arrays.
+
+    precision "reference" interpolates as the reference does (f32); "fp64"
+    interpolates in f64 (the differentiable mode of mantaray_amd.adjoint).
 
     t (n+1,), x/y/kx/ky (n+1, n_rays) step-major with NaN past len[r] (only
     with trajectories=True), len (n_rays,) rows per ray including the
@@ -71,7 +75,7 @@ def trace_many(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
     outs = [np.empty((n + 1, nr)) if trajectories else None for _ in range(4)]
     ln = np.empty(nr, dtype=np.int32)
     end = np.empty((nr, 4))
-    o = _opts(math, solout, block_size=block_size, device=device)
+    o = _opts(math, solout, precision, block_size=block_size, device=device)
     C.check(C.lib().mr_trace_many(
         bathymetry.handle, current.handle, nr, *[a.ctypes.data for a in ics],
         float(t0), float(t_end), float(dt), ctypes.byref(o), t.ctypes.data,
@@ -86,7 +90,8 @@ def trace_many(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
 def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
                       t0: float = 0.0, t_end: float = 1.0, dt: float = 1.0, *,
                       math: str = "fast", solout: str = "state", trajectories: bool = True,
-                      block_size: int = 0, out: dict | None = None, stream=None) -> dict:
+                      block_size: int = 0, out: dict | None = None, stream=None,
+                      precision: str = "reference") -> dict:
     """Trace rays whose initial conditions are float64 torch tensors on a GPU.
 
     Outputs are torch tensors on the same device (or the preallocated ones in
@@ -111,7 +116,7 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
                       if trajectories else None)
     if stream is None:
         stream = torch.cuda.current_stream(dev)
-    o = _opts(math, solout, block_size=block_size, memory=C.MR_MEM_DEVICE,
+    o = _opts(math, solout, precision, block_size=block_size, memory=C.MR_MEM_DEVICE,
               device=dev.index if dev.index is not None else -1,
               stream=ctypes.c_void_p(stream.cuda_stream))
 
@@ -126,11 +131,12 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
     return out
 
 
-def rhs(bathymetry: Bathymetry, current: Current, states, *, math: str = "fast") -> np.ndarray:
+def rhs(bathymetry: Bathymetry, current: Current, states, *, math: str = "fast",
+        precision: str = "reference") -> np.ndarray:
     """WaveRayPath::system at states (N, 4) -> derivatives (N, 4), all-NaN on error."""
     s = np.ascontiguousarray(np.asarray(states, dtype=np.float64).reshape(-1, 4))
     ds = np.empty_like(s)
-    o = _opts(math)
+    o = _opts(math, precision=precision)
     C.check(C.lib().mr_rhs_many(bathymetry.handle, current.handle, s.shape[0], s.ctypes.data,
                                 ds.ctypes.data, ctypes.byref(o)))
     return ds

@@ -1,0 +1,392 @@
+// mr_adjoint.hip -- gfx950 kernels of the fp64-interpolation mode and of its
+// discrete adjoint.
+//
+//   trace64_kernel   ManyRays::trace_many (src/ray.rs:98-127) with every
+//                    field query in f64 (mr_device64.h); optionally stores the
+//                    state every `stride` steps (checkpoints).
+//   adjoint_kernel   reverse sweep of the discrete adjoint of the RK4 loop:
+//                    gradients of a loss on the rows (or the last row) with
+//                    respect to the initial states and to the gridded depth,
+//                    u and v values.
+//   rhs64_kernel     WaveRayPath::system in fp64 mode at a batch of states.
+//   build_m64        per-cell change of basis (m01, m10) of interpolator.rs
+//                    :59-72 in f64.
+//   fill             constant fill (gradient grids start at zero).
+//
+// The adjoint is that of the discrete map the forward computes (ode_solvers
+// Rk4::step, k1 .. k4 and (((k1 + 2 k2) + 2 k3) + k4) dt/6), so its
+// gradients are those of the computed trajectories, not of the ODE.  Each
+// lane owns one ray and walks its checkpointed segments backwards: it
+// recomputes the segment's states and stage derivatives from the checkpoint
+// with the forward's own step code (bitwise the forward's values) into a
+// per-lane column of the work buffer, then runs the step adjoints in reverse
+// order over them.
+#include "mr_kernels.h"
+#include "mr_kernel_common.h"
+
+namespace mr {
+
+// One RK4 step in fp64 mode from s: the first three stage derivatives k and
+// the new state o.  The arithmetic is that of trace_kernel's step.
+template <int BK, int CK, int MATH>
+__device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs &BP,
+                                           const Grid64View &C, const Grid64Ptrs &CP,
+                                           const Fields64 &F, const FastConsts &K, double dt,
+                                           double half, double sixth, const double s[4],
+                                           double k[3][4], double o[4])
+{
+    double sx = s[0], sy = s[1], skx = s[2], sky = s[3];
+    double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const double hs = st < 2 ? half : dt;
+        double dx, dy, dkx, dky;
+        rhs64<BK, CK, MATH>(B, BP, C, CP, F, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        if (st < 3) {
+            k[st][0] = dx; k[st][1] = dy; k[st][2] = dkx; k[st][3] = dky;
+        }
+        if (st == 0) {
+            ax = dx; ay = dy; akx = dkx; aky = dky;
+        } else if (st < 3) {
+            ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
+            akx = __builtin_fma(dkx, 2.0, akx); aky = __builtin_fma(dky, 2.0, aky);
+        } else {
+            ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
+        }
+        sx = s[0] + dx * hs; sy = s[1] + dy * hs;
+        skx = s[2] + dkx * hs; sky = s[3] + dky * hs;
+    }
+    o[0] = s[0] + ax * sixth; o[1] = s[1] + ay * sixth;
+    o[2] = s[2] + akx * sixth; o[3] = s[3] + aky * sixth;
+}
+
+// ---------------------------------------------------------------------------
+// fp64 trace: the loop of trace_kernel (mr_kernels.hip) with rhs64 and
+// without record prefetch (every query reads its corners).  Checkpoint c
+// holds the state at row c * stride, stored when the step from that row
+// starts; a ray's checkpoints up to its last live step are always written.
+
+template <int BK, int CK, int MATH, int OUT>
+__global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
+{
+    if ((int)blockIdx.x == a.ray_blocks) {
+        if (threadIdx.x == 0) time_column(a.t0, a.dt, a.n_steps, a.t_out);
+        return;
+    }
+    const int64_t nr = a.n_rays;
+    const int64_t r = __builtin_elementwise_min((int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                                nr - 1);
+    double s[4] = {a.x0[r], a.y0[r], a.kx0[r], a.ky0[r]};
+    if (OUT == 1) {
+        a.xo[r] = s[0]; a.yo[r] = s[1]; a.kxo[r] = s[2]; a.kyo[r] = s[3];
+    } else if (OUT == 2) {
+        if (a.xo) a.xo[r] = s[0];
+        if (a.yo) a.yo[r] = s[1];
+        if (a.kxo) a.kxo[r] = s[2];
+        if (a.kyo) a.kyo[r] = s[3];
+    }
+    const Grid64View B = load_view<Grid64View>(a.B);
+    const Grid64View C = load_view<Grid64View>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    const double half = a.half, dt = a.dt, sixth = a.sixth;
+    bool alive = true;
+    int64_t len = 1;
+    int64_t s_i = 1;
+    int until_ckpt = 0;                // steps to the next checkpoint
+    double *ck = a.ckpt ? a.ckpt + r : nullptr;
+    for (; s_i <= a.n_steps; ++s_i) {
+        if (!__any(alive)) break;
+        if (ck) {
+            if (until_ckpt == 0) {
+                ck[0] = s[0]; ck[nr] = s[1]; ck[2 * nr] = s[2]; ck[3 * nr] = s[3];
+                ck += 4 * nr;
+                until_ckpt = a.stride;
+            }
+            --until_ckpt;
+        }
+        double o[4] = {MR_NAN, MR_NAN, MR_NAN, MR_NAN};
+        if (alive) {
+            double k[3][4], n[4];
+            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, dt, half, sixth, s, k, n);
+            if (a.solout == 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
+                alive = false;         // the ray ended on the row before
+            } else {
+                o[0] = n[0]; o[1] = n[1]; o[2] = n[2]; o[3] = n[3];
+                s[0] = n[0]; s[1] = n[1]; s[2] = n[2]; s[3] = n[3];
+                len = s_i + 1;
+                if (all_nan4(n[0], n[1], n[2], n[3])) alive = false;
+            }
+        }
+        const int64_t off = s_i * nr + r;
+        if (OUT == 1) {
+            __builtin_nontemporal_store(o[0], a.xo + off);
+            __builtin_nontemporal_store(o[1], a.yo + off);
+            __builtin_nontemporal_store(o[2], a.kxo + off);
+            __builtin_nontemporal_store(o[3], a.kyo + off);
+        } else if (OUT == 2) {
+            if (a.xo) a.xo[off] = o[0];
+            if (a.yo) a.yo[off] = o[1];
+            if (a.kxo) a.kxo[off] = o[2];
+            if (a.kyo) a.kyo[off] = o[3];
+        }
+    }
+    if (OUT != 0) {
+        for (; s_i <= a.n_steps; ++s_i) {
+            const int64_t off = s_i * nr + r;
+            if (OUT == 1 || a.xo) __builtin_nontemporal_store(MR_NAN, a.xo + off);
+            if (OUT == 1 || a.yo) __builtin_nontemporal_store(MR_NAN, a.yo + off);
+            if (OUT == 1 || a.kxo) __builtin_nontemporal_store(MR_NAN, a.kxo + off);
+            if (OUT == 1 || a.kyo) __builtin_nontemporal_store(MR_NAN, a.kyo + off);
+        }
+    }
+    if (a.len_out) a.len_out[r] = (int32_t)len;
+    if (a.end_out) {
+        a.end_out[4 * r + 0] = s[0]; a.end_out[4 * r + 1] = s[1];
+        a.end_out[4 * r + 2] = s[2]; a.end_out[4 * r + 3] = s[3];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of one RK4 step.  With lam the cotangent of the new state
+// y' = y + (((k1 + 2 k2) + 2 k3) + k4) dt/6 and the stage points
+// Y1 = y, Y2 = y + k1 dt/2, Y3 = y + k2 dt/2, Y4 = y + k3 dt:
+//   kb4 = kb1 = lam dt/6, kb3 = kb2 = 2 lam dt/6
+//   z4 = J(Y4)^T kb4;  kb3 += z4 dt
+//   z3 = J(Y3)^T kb3;  kb2 += z3 dt/2
+//   z2 = J(Y2)^T kb2;  kb1 += z2 dt/2
+//   z1 = J(Y1)^T kb1
+//   lam <- lam + z1 + z2 + z3 + z4
+// and the field-value cotangents of the four products go to the grids.
+template <int BK, int CK>
+__device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid64Ptrs &BP,
+                                                 const Grid64View &C, const Grid64Ptrs &CP,
+                                                 const Fields64 &F, const Grads64 &Gr,
+                                                 Accs64 &acc, const FastConsts &K, double dt,
+                                                 double half, double sixth, const double s[4],
+                                                 const double k[3][4], double lam[4])
+{
+    double kb[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double w = lam[q] * sixth;
+        kb[0][q] = w; kb[3][q] = w;
+        kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
+    }
+#pragma unroll
+    for (int st = 3; st >= 0; --st) {
+        double Y[4];
+        if (st == 0) {
+            Y[0] = s[0]; Y[1] = s[1]; Y[2] = s[2]; Y[3] = s[3];
+        } else {
+            const double hs = st < 3 ? half : dt;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Y[q] = s[q] + k[st - 1][q] * hs;
+        }
+        double z[4] = {0.0, 0.0, 0.0, 0.0};
+        rhs64_vjp<BK, CK>(B, BP, C, CP, F, Gr, acc, K, Y[0], Y[1], Y[2], Y[3], kb[st][0],
+                          kb[st][1], kb[st][2], kb[st][3], z[0], z[1], z[2], z[3]);
+        if (st > 0) {
+            const double hs = st < 3 ? half : dt;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) kb[st - 1][q] += z[q] * hs;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lam[q] += z[q];
+    }
+}
+
+// The reverse sweep.  Row m is the ray's last finite row: len-1, or len-2
+// when the last row is the all-NaN one (a ray that left the domain, the
+// row `end` holds).  Cotangents: g_end on row len-1 when it is finite, g_x ..
+// g_ky on rows 0..m; none reaches a NaN row.  Lanes past n_rays return at
+// once (no barriers follow).
+template <int BK, int CK, int MATH>
+__global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
+{
+    const int64_t nr = a.n_rays;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const Grid64View B = load_view<Grid64View>(a.B);
+    const Grid64View C = load_view<Grid64View>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    const double dt = a.dt, half = a.half, sixth = a.sixth;
+    const int stride = a.stride;
+    const int64_t ld = a.work_ld;
+    double *wk = a.work + r;
+    const bool dead = all_nan4(a.end[4 * r], a.end[4 * r + 1], a.end[4 * r + 2],
+                               a.end[4 * r + 3]);
+    const int64_t m = (int64_t)a.len[r] - (dead ? 2 : 1);
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};
+    if (m >= 0) {
+        if (a.g_end && !dead) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
+        }
+        const int64_t o = m * nr + r;
+        if (a.g_x) lam[0] += a.g_x[o];
+        if (a.g_y) lam[1] += a.g_y[o];
+        if (a.g_kx) lam[2] += a.g_kx[o];
+        if (a.g_ky) lam[3] += a.g_ky[o];
+    }
+    Accs64 acc;
+    for (int64_t c = m >= 1 ? (m - 1) / stride : -1; c >= 0; --c) {
+        const int64_t i0 = c * stride;
+        const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
+        // recompute the segment's states and stage derivatives
+        const double *cp = a.ckpt + (c * 4) * nr + r;
+        double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
+        for (int j = 0; j < nseg; ++j) {
+            double k[3][4], n[4];
+            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, dt, half, sixth, s, k, n);
+            double *w = wk + (int64_t)j * 16 * ld;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                w[q * ld] = s[q];
+                w[(4 + q) * ld] = k[0][q];
+                w[(8 + q) * ld] = k[1][q];
+                w[(12 + q) * ld] = k[2][q];
+                s[q] = n[q];
+            }
+        }
+        // step adjoints in reverse
+        for (int j = nseg - 1; j >= 0; --j) {
+            const double *w = wk + (int64_t)j * 16 * ld;
+            double k[3][4], y[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                y[q] = w[q * ld];
+                k[0][q] = w[(4 + q) * ld];
+                k[1][q] = w[(8 + q) * ld];
+                k[2][q] = w[(12 + q) * ld];
+            }
+            rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, a.grads, acc, K, dt, half, sixth, y,
+                                     k, lam);
+            const int64_t o = (i0 + j) * nr + r;
+            if (a.g_x) lam[0] += a.g_x[o];
+            if (a.g_y) lam[1] += a.g_y[o];
+            if (a.g_kx) lam[2] += a.g_kx[o];
+            if (a.g_ky) lam[3] += a.g_ky[o];
+        }
+    }
+    if (a.g_x0) a.g_x0[r] = lam[0];
+    if (a.g_y0) a.g_y0[r] = lam[1];
+    if (a.g_kx0) a.g_kx0[r] = lam[2];
+    if (a.g_ky0) a.g_ky0[r] = lam[3];
+    if (BK == BK_GRID && a.grads.depth) acc_flush(acc.h, a.grads.depth, (uint32_t)B.nx);
+    if (CK == CK_GRID) {
+        if (a.grads.u) acc_flush(acc.u, a.grads.u, (uint32_t)C.nx);
+        if (a.grads.v) acc_flush(acc.v, a.grads.v, (uint32_t)C.nx);
+    }
+}
+
+// WaveRayPath::system in fp64 mode at a batch of states
+template <int BK, int CK, int MATH>
+__global__ void __launch_bounds__(256) rhs64_kernel(Rhs64Args a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Grid64View B = load_view<Grid64View>(a.B);
+    const Grid64View C = load_view<Grid64View>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    double dx, dy, dkx, dky;
+    rhs64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2],
+                        a.s[4 * i + 3], dx, dy, dkx, dky);
+    a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
+}
+
+// per-cell (m01, m10) of interpolator.rs:59-72 in f64, on the corners
+// A = (x_i, y_j), B = (x_i, y_j+1), D = (x_i+1, y_j); NaN marks det == 0
+__global__ void build_m64_kernel(M64BuildArgs a)
+{
+    const int64_t ncx = a.nx - 1;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncx * (a.ny - 1)) return;
+    const int64_t i = c % ncx, j = c / ncx;
+    const double Ax = a.ax[i], Ay = a.ay[j];
+    const double btx = Ax - Ax, bty = a.ay[j + 1] - Ay;
+    const double dtx = a.ax[i + 1] - Ax, dty = Ay - Ay;
+    const double det = (btx * dty) - (dtx * bty);
+    a.m[c] = det == 0.0 ? make_double2(MR_NAN, MR_NAN)
+                        : make_double2(-(dtx / det), -(bty / det));
+}
+
+__global__ void fill_kernel(double *p, int64_t n, double v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,
+                          hipStream_t stream)
+{
+    if (a.n_rays <= 0) return hipSuccess;
+    if (a.ray_blocks != (int)((a.n_rays + block - 1) / block)) return hipErrorInvalidValue;
+    if (a.ckpt && a.stride <= 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
+    const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
+                     (a.kyo != nullptr);
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using KK = decltype(k);
+        if (nout == 0)
+            hipLaunchKernelGGL((trace64_kernel<KK::bk, KK::ck, KK::math, 0>), grid, dim3(block),
+                               0, stream, a);
+        else if (nout == 4)
+            hipLaunchKernelGGL((trace64_kernel<KK::bk, KK::ck, KK::math, 1>), grid, dim3(block),
+                               0, stream, a);
+        else
+            hipLaunchKernelGGL((trace64_kernel<KK::bk, KK::ck, KK::math, 2>), grid, dim3(block),
+                               0, stream, a);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int block,
+                          hipStream_t stream)
+{
+    if (a.n_rays <= 0) return hipSuccess;
+    if (a.stride <= 0 || a.work_ld < a.n_rays) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((a.n_rays + block - 1) / block));
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using KK = decltype(k);
+        hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(block), 0,
+                           stream, a);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs64(const Rhs64Args &a, int bk, int ck, int math, hipStream_t stream)
+{
+    if (a.n <= 0) return hipSuccess;
+    const int block = 256;
+    const dim3 grid((unsigned)((a.n + block - 1) / block));
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using KK = decltype(k);
+        hipLaunchKernelGGL((rhs64_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(block), 0, stream,
+                           a);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_build_m64(const M64BuildArgs &a, hipStream_t stream)
+{
+    const int64_t ncell = (a.nx - 1) * (a.ny - 1);
+    if (ncell <= 0) return hipSuccess;
+    const int block = 256;
+    hipLaunchKernelGGL(build_m64_kernel, dim3((unsigned)((ncell + block - 1) / block)),
+                       dim3(block), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace mr

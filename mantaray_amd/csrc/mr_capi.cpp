@@ -45,6 +45,11 @@ struct DevTables {
     void *view = nullptr;          // BathyView or CurrentView in device memory
     BathyPtrs bp{};                // device pointers of a bathymetry
     CurrentPtrs cp{};              // device pointers of a current
+    // fp64-interpolation mode, built on first use
+    void *view64 = nullptr;        // Grid64View in device memory
+    Grid64Ptrs p64{};
+    const double *va = nullptr;    // the field's own values: depth, or u
+    const double *vb = nullptr;    // v
     ~DevTables()
     {
         for (void *p : allocs) (void)hipFree(p);
@@ -432,6 +437,72 @@ int field_tables(const mr_field *cf, const DevTables **out)
     return MR_OK;
 }
 
+// the fp64-mode view of a gridded or analytic field (mr_device64.h)
+int build64(const mr_field &f, DevTables &t)
+{
+    Grid64View v;
+    std::memset(&v, 0, sizeof v);
+    const bool bathy = is_bathy(f.kind);
+    v.kind = bathy ? f.kind : (f.kind == MR_CURRENT_CONSTANT ? CK_CONSTANT : CK_GRID);
+    if (bathy) {
+        v.h = (double)f.h;
+        v.h0 = (double)f.h0; v.x0 = (double)f.x0; v.y0 = (double)f.y0;
+        v.dhdx = (double)f.dhdx; v.dhdy = (double)f.dhdy;
+        v.an = (int)f.an;
+        t.p64.arr = t.bp.arr;
+        t.va = t.bp.raw;
+    } else {
+        v.u = f.u; v.v = f.v;
+        t.va = t.cp.raw_u;
+        t.vb = t.cp.raw_v;
+    }
+    if (f.kind == MR_BATHY_GRID || f.kind == MR_CURRENT_GRID) {
+        std::vector<double> ax(f.nx), ay(f.ny);
+        for (int64_t i = 0; i < f.nx; i++) ax[(size_t)i] = bathy ? (double)f.xf[(size_t)i] : f.xd[(size_t)i];
+        for (int64_t i = 0; i < f.ny; i++) ay[(size_t)i] = bathy ? (double)f.yf[(size_t)i] : f.yd[(size_t)i];
+        v.nx = (int)f.nx; v.ny = (int)f.ny; v.ncx = (int)(f.nx - 1);
+        v.a0x = ax[0]; v.a0y = ay[0];
+        v.sx = std::fabs(ax[1] - ax[0]); v.sy = std::fabs(ay[1] - ay[0]);
+        v.inv_sx = 1.0 / v.sx; v.inv_sy = 1.0 / v.sy;
+        v.nxm1 = (double)(f.nx - 1); v.nym1 = (double)(f.ny - 1);
+        v.xspace = ax[1] - ax[0]; v.yspace = ay[1] - ay[0];
+        v.rxs = 1.0 / v.xspace; v.rys = 1.0 / v.yspace;
+        v.oob_from = (uint32_t)(bathy ? first_oob_cell(f.nx, f.ny, (int64_t)f.a.size())
+                                      : std::min(first_oob_cell(f.nx, f.ny, (int64_t)f.a.size()),
+                                                 first_oob_cell(f.nx, f.ny, (int64_t)f.b.size())));
+        double *dax, *day;
+        double2 *dm;
+        int rc = dev_upload(t, ax.data(), ax.size(), &dax);
+        if (!rc) rc = dev_upload(t, ay.data(), ay.size(), &day);
+        if (!rc) rc = dev_alloc(t, (size_t)((f.nx - 1) * (f.ny - 1)), &dm);
+        if (rc) return rc;
+        M64BuildArgs ma;
+        ma.nx = f.nx; ma.ny = f.ny; ma.ax = dax; ma.ay = day; ma.m = dm;
+        MR_HIP_TRY(launch_build_m64(ma, 0));
+        t.p64.ax = dax; t.p64.ay = day; t.p64.m = dm;
+    }
+    Grid64View *dv;
+    int rc = dev_upload(t, &v, 1, &dv);
+    if (rc) return rc;
+    MR_HIP_TRY(hipDeviceSynchronize());
+    t.view64 = dv;
+    return MR_OK;
+}
+
+// field_tables plus the fp64-mode view
+int field_tables64(const mr_field *cf, const DevTables **out)
+{
+    const DevTables *t0;
+    int rc = field_tables(cf, &t0);
+    if (rc) return rc;
+    mr_field *f = const_cast<mr_field *>(cf);
+    std::lock_guard<std::mutex> lk(f->mu);
+    DevTables *t = const_cast<DevTables *>(t0);
+    if (!t->view64 && (rc = build64(*f, *t))) return rc;
+    *out = t;
+    return MR_OK;
+}
+
 int check_pair(const mr_field *b, const mr_field *c)
 {
     if (!b || !c) return fail(MR_ERR_INVALID_ARGUMENT, "null field handle");
@@ -448,8 +519,8 @@ int resolve_options(const mr_options *opt, mr_options &o)
             return fail(MR_ERR_INVALID_ARGUMENT, "mr_options.struct_size too small");
         o = *opt;
     }
-    if (o.precision != MR_PREC_REFERENCE)
-        return fail(MR_ERR_INVALID_ARGUMENT, "unsupported precision mode");
+    if (o.precision != MR_PREC_REFERENCE && o.precision != MR_PREC_FP64_INTERP)
+        return fail(MR_ERR_INVALID_ARGUMENT, "unknown precision mode");
     if (o.math != MR_MATH_FAST && o.math != MR_MATH_LITERAL)
         return fail(MR_ERR_INVALID_ARGUMENT, "unknown math mode");
     if (o.solout != MR_SOLOUT_STATE && o.solout != MR_SOLOUT_NEXT_DERIV)
@@ -478,6 +549,85 @@ int tuned_block(int64_t n_rays)
     return (int)(64 * waves);
 }
 
+// mr_rhs_many in fp64 mode
+int rhs64_many(const mr_field *bathy, const mr_field *current, int64_t n, const double *s,
+               double *ds, const mr_options &o)
+{
+    const DevTables *bt, *ct;
+    int rc;
+    if ((rc = field_tables64(bathy, &bt)) || (rc = field_tables64(current, &ct))) return rc;
+    Rhs64Args a;
+    a.B = bt->view64;
+    a.C = ct->view64;
+    a.bp = bt->p64;
+    a.cp = ct->p64;
+    a.f.depth = bt->va;
+    a.f.u = ct->va;
+    a.f.v = ct->vb;
+    a.n = n;
+    if (o.memory == MR_MEM_DEVICE) {
+        a.s = s; a.ds = ds;
+        MR_HIP_TRY(launch_rhs64(a, bathy_kind_of(bathy), current_kind_of(current), o.math,
+                                (hipStream_t)o.stream));
+        return MR_OK;
+    }
+    DevTables tmp;
+    double *d_s, *d_ds;
+    if ((rc = dev_alloc(tmp, 4 * (size_t)n, &d_s)) || (rc = dev_alloc(tmp, 4 * (size_t)n, &d_ds)))
+        return rc;
+    if (n) MR_HIP_TRY(hipMemcpy(d_s, s, 4 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    a.s = d_s; a.ds = d_ds;
+    MR_HIP_TRY(launch_rhs64(a, bathy_kind_of(bathy), current_kind_of(current), o.math, 0));
+    MR_HIP_TRY(hipDeviceSynchronize());
+    if (n) MR_HIP_TRY(hipMemcpy(ds, d_ds, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return MR_OK;
+}
+
+// the fp64-mode trace, all pointers device pointers, enqueued on stream;
+// depth, u, v replace the fields' own values when not null; ckpt (with
+// stride > 0) receives the checkpoints
+int trace64_device(const mr_field *bathy, const mr_field *current, const double *depth,
+                   const double *u, const double *v, int64_t n_rays, const double *x0,
+                   const double *y0, const double *kx0, const double *ky0, double t0, int64_t n,
+                   double dt, int32_t stride, const mr_options &o, hipStream_t stream,
+                   double *ckpt, double *t_out, double *x_out, double *y_out, double *kx_out,
+                   double *ky_out, int32_t *len_out, double *end_out)
+{
+    const DevTables *bt, *ct;
+    int rc = field_tables64(bathy, &bt);
+    if (!rc) rc = field_tables64(current, &ct);
+    if (rc) return rc;
+    if (t_out && n_rays == 0) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
+    Trace64Args a;
+    std::memset(&a, 0, sizeof a);
+    a.B = bt->view64;
+    a.C = ct->view64;
+    a.bp = bt->p64;
+    a.cp = ct->p64;
+    a.f.depth = depth ? depth : bt->va;
+    a.f.u = u ? u : ct->va;
+    a.f.v = v ? v : ct->vb;
+    a.n_rays = n_rays;
+    a.n_steps = n;
+    a.t0 = t0;
+    a.t_out = t_out;
+    a.dt = dt;
+    a.half = dt / 2.0;
+    a.sixth = dt / 6.0;
+    a.x0 = x0; a.y0 = y0; a.kx0 = kx0; a.ky0 = ky0;
+    a.xo = x_out; a.yo = y_out; a.kxo = kx_out; a.kyo = ky_out;
+    a.len_out = len_out;
+    a.end_out = end_out;
+    a.solout = o.solout;
+    a.stride = stride;
+    a.ckpt = ckpt;
+    const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    a.ray_blocks = (int)((n_rays + block - 1) / block);
+    MR_HIP_TRY(launch_trace64(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
+                              stream));
+    return MR_OK;
+}
+
 // the trace itself, all pointers device pointers, enqueued on stream
 int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                  const double *x0, const double *y0, const double *kx0, const double *ky0,
@@ -485,6 +635,10 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                  double *t_out, double *x_out, double *y_out, double *kx_out, double *ky_out,
                  int32_t *len_out, double *end_out)
 {
+    if (o.precision == MR_PREC_FP64_INTERP)
+        return trace64_device(bathy, current, nullptr, nullptr, nullptr, n_rays, x0, y0, kx0, ky0,
+                              t0, n, dt, 0, o, stream, nullptr, t_out, x_out, y_out, kx_out,
+                              ky_out, len_out, end_out);
     const DevTables *bt, *ct;
     int rc = field_tables(bathy, &bt);
     if (!rc) rc = field_tables(current, &ct);
@@ -817,6 +971,7 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     if (n < 0 || (n && (!s || !ds))) return fail(MR_ERR_INVALID_ARGUMENT, "invalid arrays");
     DeviceGuard g(o.device);
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    if (o.precision == MR_PREC_FP64_INTERP) return rhs64_many(bathy, current, n, s, ds, o);
     const DevTables *bt, *ct;
     if ((rc = field_tables(bathy, &bt)) || (rc = field_tables(current, &ct))) return rc;
     RhsArgs a;
@@ -840,6 +995,120 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     MR_HIP_TRY(launch_rhs(a, bathy_kind_of(bathy), current_kind_of(current), o.math, 0));
     MR_HIP_TRY(hipDeviceSynchronize());
     if (n) MR_HIP_TRY(hipMemcpy(ds, d_ds, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return MR_OK;
+}
+
+int mr_checkpoint_count(double t0, double t_end, double dt, int32_t stride, int64_t *count)
+{
+    if (!count) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
+    if (stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "checkpoint stride must be positive");
+    int64_t n;
+    int rc = mr_num_steps(t0, t_end, dt, &n);
+    if (rc) return rc;
+    *count = std::max<int64_t>(1, (n + stride - 1) / stride);
+    return MR_OK;
+}
+
+int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes)
+{
+    if (!bytes) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
+    if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
+    *bytes = (int64_t)stride * 16 * n_rays * (int64_t)sizeof(double);
+    return MR_OK;
+}
+
+int mr_trace_fp64(const mr_field *bathy, const mr_field *current, const double *depth,
+                  const double *u, const double *v, int64_t n_rays, const double *x0,
+                  const double *y0, const double *kx0, const double *ky0, double t0,
+                  double t_end, double dt, int32_t stride, const mr_options *opt, double *ckpt,
+                  double *t_out, double *x_out, double *y_out, double *kx_out, double *ky_out,
+                  int32_t *len_out, double *end_out)
+{
+    int rc = check_pair(bathy, current);
+    if (rc) return rc;
+    mr_options o;
+    if ((rc = resolve_options(opt, o))) return rc;
+    if (o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "mr_trace_fp64 takes device memory (MR_MEM_DEVICE)");
+    if (n_rays < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n_rays > 0 && (!x0 || !y0 || !kx0 || !ky0))
+        return fail(MR_ERR_INVALID_ARGUMENT, "null initial conditions");
+    if (ckpt && stride <= 0)
+        return fail(MR_ERR_INVALID_ARGUMENT, "checkpoint stride must be positive");
+    int64_t n;
+    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    return trace64_device(bathy, current, depth, u, v, n_rays, x0, y0, kx0, ky0, t0, n, dt,
+                          ckpt ? stride : 0, o, (hipStream_t)o.stream, ckpt, t_out, x_out, y_out,
+                          kx_out, ky_out, len_out, end_out);
+}
+
+int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double *depth,
+                    const double *u, const double *v, int64_t n_rays, double t0, double t_end,
+                    double dt, int32_t stride, const mr_options *opt, const double *ckpt,
+                    const int32_t *len, const double *end, const double *g_end,
+                    const double *g_x, const double *g_y, const double *g_kx, const double *g_ky,
+                    void *work, int64_t work_bytes, double *g_x0, double *g_y0, double *g_kx0,
+                    double *g_ky0, double *g_depth, double *g_u, double *g_v)
+{
+    int rc = check_pair(bathy, current);
+    if (rc) return rc;
+    mr_options o;
+    if ((rc = resolve_options(opt, o))) return rc;
+    if (o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "mr_adjoint_many takes device memory (MR_MEM_DEVICE)");
+    if (o.solout != MR_SOLOUT_STATE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "the adjoint supports MR_SOLOUT_STATE only");
+    if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
+    if (g_depth && bathy->kind != MR_BATHY_GRID)
+        return fail(MR_ERR_INVALID_ARGUMENT, "depth gradient requested of a non-gridded bathymetry");
+    if ((g_u || g_v) && current->kind != MR_CURRENT_GRID)
+        return fail(MR_ERR_INVALID_ARGUMENT, "current gradient requested of a non-gridded current");
+    int64_t need = 0;
+    mr_adjoint_workspace_size(n_rays, stride, &need);
+    if (n_rays > 0 && (!ckpt || !len || !end || !work))
+        return fail(MR_ERR_INVALID_ARGUMENT, "null checkpoint, length, end or work buffer");
+    if (work_bytes < need) return fail(MR_ERR_INVALID_ARGUMENT, "work buffer too small");
+    int64_t n;
+    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    const DevTables *bt, *ct;
+    if ((rc = field_tables64(bathy, &bt)) || (rc = field_tables64(current, &ct))) return rc;
+    hipStream_t stream = (hipStream_t)o.stream;
+    if (g_depth) MR_HIP_TRY(launch_fill(g_depth, (int64_t)bathy->a.size(), 0.0, stream));
+    if (g_u) MR_HIP_TRY(launch_fill(g_u, (int64_t)current->a.size(), 0.0, stream));
+    if (g_v) MR_HIP_TRY(launch_fill(g_v, (int64_t)current->b.size(), 0.0, stream));
+    AdjointArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.B = bt->view64;
+    a.C = ct->view64;
+    a.bp = bt->p64;
+    a.cp = ct->p64;
+    a.f.depth = depth ? depth : bt->va;
+    a.f.u = u ? u : ct->va;
+    a.f.v = v ? v : ct->vb;
+    a.n_rays = n_rays;
+    a.n_steps = n;
+    a.dt = dt;
+    a.half = dt / 2.0;
+    a.sixth = dt / 6.0;
+    a.stride = stride;
+    a.ckpt = ckpt;
+    a.len = len;
+    a.end = end;
+    a.g_end = g_end;
+    a.g_x = g_x; a.g_y = g_y; a.g_kx = g_kx; a.g_ky = g_ky;
+    a.work = static_cast<double *>(work);
+    a.work_ld = n_rays;
+    a.g_x0 = g_x0; a.g_y0 = g_y0; a.g_kx0 = g_kx0; a.g_ky0 = g_ky0;
+    a.grads.depth = g_depth;
+    a.grads.u = g_u;
+    a.grads.v = g_v;
+    const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
+                              stream));
     return MR_OK;
 }
 

@@ -490,18 +490,16 @@ __device__ __forceinline__ void exp_pair(double r, double &e, double &em, const 
 }
 
 // ---------------------------------------------------------------------------
-// WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150)
+// WaveRayPath::odes (wave_ray_path.rs:118-150) from the field values at the
+// point: depth h (NaN where the bathymetry errs), its gradient, the current
+// and its gradients, and okc = false where the current errs.
 
-template <int BK, int CK, int MATH>
-__device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
-                                    const CurrentView &C, const CurrentPtrs &CP,
-                                    CurrentSlot &cs, const FastConsts &K, double x, double y,
-                                    double kx, double ky, double &dx, double &dy, double &dkx,
-                                    double &dky)
+template <int MATH>
+__device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, double hy,
+                                     bool okc, double u, double v, double ux, double uy,
+                                     double vx, double vy, double kx, double ky, double &dx,
+                                     double &dy, double &dkx, double &dky)
 {
-    double hx, hy, u, v, ux, uy, vx, vy;
-    double h = bathy_query<BK>(B, BP, bs, x, y, hx, hy);             // odes :120-127
-    const bool okc = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);   // :130
     if (MATH == MATH_LITERAL) {
         // A NaN depth (a provider error, or a NaN target or datum) makes
         // every derivative NaN through group_velocity and dkdt_bathy, as an
@@ -559,6 +557,20 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
         dkx = (c * hx - kx * ux) - ky * vx;
         dky = (c * hy - kx * uy) - ky * vy;
     }
+}
+
+// WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150)
+template <int BK, int CK, int MATH>
+__device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
+                                    const CurrentView &C, const CurrentPtrs &CP,
+                                    CurrentSlot &cs, const FastConsts &K, double x, double y,
+                                    double kx, double ky, double &dx, double &dy, double &dkx,
+                                    double &dky)
+{
+    double hx, hy, u, v, ux, uy, vx, vy;
+    const double h = bathy_query<BK>(B, BP, bs, x, y, hx, hy);       // odes :120-127
+    const bool okc = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);   // :130
+    odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
 }
 
 __device__ __forceinline__ bool all_nan4(double a, double b, double c, double d)

@@ -1,7 +1,7 @@
 // mr_kernels.h -- kernel argument blocks and launchers (host-visible).
 #pragma once
 
-#include "mr_device.h"
+#include "mr_device64.h"
 
 namespace mr {
 
@@ -55,11 +55,71 @@ struct CurrentBuildArgs {
     float2 *mtab;                      // per-cell change of basis
 };
 
+// fp64-interpolation trace (MR_PREC_FP64_INTERP), optionally writing the
+// state at every stride-th step (checkpoints for the adjoint sweep).
+struct Trace64Args {
+    const void *B, *C;                 // Grid64View in device memory
+    Grid64Ptrs bp, cp;
+    Fields64 f;                        // field values (depth, u, v)
+    int64_t n_rays, n_steps;
+    double t0, dt, half, sixth;
+    double *t_out;
+    int ray_blocks;
+    const double *x0, *y0, *kx0, *ky0;
+    double *xo, *yo, *kxo, *kyo;
+    int32_t *len_out;
+    double *end_out;
+    int solout;
+    int stride;                        // checkpoint every stride steps
+    double *ckpt;                      // [n_ckpt][4][n_rays], may be null
+};
+
+// Reverse sweep of the discrete RK4 adjoint.
+struct AdjointArgs {
+    const void *B, *C;
+    Grid64Ptrs bp, cp;
+    Fields64 f;
+    int64_t n_rays, n_steps;
+    double dt, half, sixth;
+    int stride;
+    const double *ckpt;                // [n_ckpt][4][n_rays]
+    const int32_t *len;                // rows of each ray
+    const double *end;                 // last row of each ray (4 per ray)
+    const double *g_end;               // cotangent of the last row (4 per ray), may be null
+    const double *g_x, *g_y, *g_kx, *g_ky;   // cotangents of the rows, step-major, may be null
+    double *work;                      // [stride][16][work_ld]
+    int64_t work_ld;                   // >= blocks * block threads
+    double *g_x0, *g_y0, *g_kx0, *g_ky0;     // may be null
+    Grads64 grads;                     // gradient grids, cleared by the caller
+};
+
+struct Rhs64Args {
+    const void *B, *C;
+    Grid64Ptrs bp, cp;
+    Fields64 f;
+    int64_t n;
+    const double *s;
+    double *ds;
+};
+
+struct M64BuildArgs {
+    int64_t nx, ny;
+    const double *ax, *ay;
+    double2 *m;
+};
+
 hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
                         hipStream_t stream);
 hipError_t launch_rhs(const RhsArgs &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_fill_time(double t0, double dt, int64_t n, double *t, hipStream_t stream);
 hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
 hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
+hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,
+                          hipStream_t stream);
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int block,
+                          hipStream_t stream);
+hipError_t launch_rhs64(const Rhs64Args &a, int bk, int ck, int math, hipStream_t stream);
+hipError_t launch_build_m64(const M64BuildArgs &a, hipStream_t stream);
+hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream);
 
 }  // namespace mr

@@ -12,61 +12,9 @@
 //                  for a trace with no rays; otherwise the trace kernel's last
 //                  workgroup writes it beside the rays.
 #include "mr_kernels.h"
-
-#include <type_traits>
+#include "mr_kernel_common.h"
 
 namespace mr {
-
-// Copy a field view from device memory into per-lane registers.  The loads
-// go through a lane-varying (always zero) offset so they are vector loads
-// and the view lives in VGPRs for the whole kernel: the scalar register file
-// cannot hold both views next to the loop's own scalars, and re-reading
-// them with scalar loads puts a load latency into every evaluation.
-template <class View>
-__device__ __forceinline__ View load_view(const void *p)
-{
-    static_assert(sizeof(View) % 4 == 0, "view size");
-    int zero;
-    __asm__ volatile("v_mov_b32 %0, 0" : "=v"(zero));
-    const int *src = reinterpret_cast<const int *>(p) + zero;
-    int buf[sizeof(View) / 4];
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(View) / 4); ++i) buf[i] = src[i];
-    View v;
-    __builtin_memcpy(&v, buf, sizeof(View));
-    return v;
-}
-
-__device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
-    1.4426950408889634,                    // 1/ln 2
-    6.93147180369123816490e-01,            // ln 2 split, upper part
-    1.90821492927058770002e-10,            // ln 2 - upper part
-    1400.0,
-    1.6059043836821614e-10,                // 1/13!
-    2.08767569878680989e-09,               // 1/12!
-    2.50521083854417188e-08,               // 1/11!
-    2.75573192239858907e-07,               // 1/10!
-    2.75573192239858907e-06,               // 1/9!
-    2.48015873015873016e-05,               // 1/8!
-    1.98412698412698413e-04,               // 1/7!
-    1.38888888888888889e-03,               // 1/6!
-    8.33333333333333333e-03,               // 1/5!
-    4.16666666666666667e-02,               // 1/4!
-    1.66666666666666667e-01,               // 1/3!
-    G,
-    0.5 * G,
-};
-
-// t[0] = t0, t[s] = t[s-1] + dt for s <= n
-__device__ __forceinline__ void time_column(double t0, double dt, int64_t n, double *t)
-{
-    double v = t0;
-    t[0] = v;
-    for (int64_t s = 1; s <= n; ++s) {
-        v = v + dt;
-        t[s] = v;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // ode_solvers 0.4.0 Rk4::step/integrate for one lane:
@@ -298,32 +246,6 @@ __global__ void build_current_kernel(CurrentBuildArgs a)
 
 // ---------------------------------------------------------------------------
 // launchers
-
-template <int BK, int CK, int MATH>
-struct Kinds {
-    static constexpr int bk = BK, ck = CK, math = MATH;
-};
-
-// Call f(Kinds<BK, CK, MATH>{}) for the run-time field kinds and math mode.
-template <class F>
-void dispatch_kinds(int bk, int ck, int math, F &&f)
-{
-    auto by_math = [&](auto b, auto c) {
-        constexpr int BK = decltype(b)::value, CK = decltype(c)::value;
-        if (math == MATH_FAST) f(Kinds<BK, CK, MATH_FAST>{});
-        else f(Kinds<BK, CK, MATH_LITERAL>{});
-    };
-    auto by_current = [&](auto b) {
-        if (ck == CK_CONSTANT) by_math(b, std::integral_constant<int, CK_CONSTANT>{});
-        else by_math(b, std::integral_constant<int, CK_GRID>{});
-    };
-    switch (bk) {
-    case BK_CONSTANT: by_current(std::integral_constant<int, BK_CONSTANT>{}); break;
-    case BK_SLOPE: by_current(std::integral_constant<int, BK_SLOPE>{}); break;
-    case BK_ARRAY: by_current(std::integral_constant<int, BK_ARRAY>{}); break;
-    default: by_current(std::integral_constant<int, BK_GRID>{}); break;
-    }
-}
 
 hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
                         hipStream_t stream)

@@ -15,6 +15,14 @@ machine with no dependence on numpy's RNG.
                    uniform interior points, period ~ U[6, 16] s, direction
                    ~ U[0, 2 pi); dt = 1 s, 5000 steps; rays that leave the
                    grid end early.
+  eddy_adjoint()   512 x 512 grid at 250 m, constant depth 50 m, current the
+                   sum of 20 Gaussian eddies (peak speed ~ U[0, 2] m/s,
+                   radius ~ U[10, 50] km, random sense, centres uniform over
+                   the grid); 16,384 rays at uniform points at least 35 km
+                   inside the edges, period ~ U[8, 14] s, direction
+                   ~ U[0, 2 pi); dt = 1 s, 2000 steps; endpoint targets
+                   launch point + U[-20, 20] km per axis.  The adjoint
+                   workload: loss sum_r |(x, y)_r(T) - target_r|^2.
 The wavenumber magnitude solves the linear dispersion relation
 sigma^2 = g k tanh(k h) (g = 9.8, sigma = 2 pi / T) at the launch depth by
 Newton's method.
@@ -122,10 +130,59 @@ def seamount_jet(n_rays: int = 65536, seed: int = 1, n_steps: int = 5000) -> dic
     }
 
 
+def eddies(x: np.ndarray, y: np.ndarray, rs: Stream, n_eddies: int = 20,
+           peak=(0.0, 2.0), radius=(10000.0, 50000.0)):
+    """u, v [j, i] of a sum of Gaussian eddies: stream function
+    psi = A exp(-r^2 / (2 R^2)), (u, v) = (-dpsi/dy, dpsi/dx), with
+    A = +-peak R e^(1/2) so each eddy's speed peaks at r = R."""
+    X, Y = np.meshgrid(x, y)
+    u = np.zeros_like(X)
+    v = np.zeros_like(X)
+    xc = rs.uniform(n_eddies, x[0], x[-1])
+    yc = rs.uniform(n_eddies, y[0], y[-1])
+    pk = rs.uniform(n_eddies, *peak)
+    R = rs.uniform(n_eddies, *radius)
+    sense = np.where(rs.uniform(n_eddies) < 0.5, -1.0, 1.0)
+    for e in range(n_eddies):
+        A = sense[e] * pk[e] * R[e] * np.exp(0.5)
+        g = np.exp(-((X - xc[e]) ** 2 + (Y - yc[e]) ** 2) / (2.0 * R[e] ** 2)) / R[e] ** 2
+        u += A * (Y - yc[e]) * g
+        v -= A * (X - xc[e]) * g
+    return u, v
+
+
+def eddy_adjoint(n_rays: int = 16384, seed: int = 3, n_steps: int = 2000) -> dict:
+    n = 512
+    dx = 250.0
+    xd = np.arange(n) * dx
+    rs = Stream(seed)
+    u, v = eddies(xd, xd, rs)
+    margin = 35000.0
+    x0 = rs.uniform(n_rays, margin, xd[-1] - margin)
+    y0 = rs.uniform(n_rays, margin, xd[-1] - margin)
+    period = rs.uniform(n_rays, 8.0, 14.0)
+    theta = rs.uniform(n_rays, 0.0, 2.0 * np.pi)
+    h = 50.0
+    k = wavenumber(2.0 * np.pi / period, np.full(n_rays, h))
+    tx = x0 + rs.uniform(n_rays, -20000.0, 20000.0)
+    ty = y0 + rs.uniform(n_rays, -20000.0, 20000.0)
+    return {
+        "name": "eddy_adjoint",
+        "bathy": {"constant": h},
+        "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
+        "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
+        "target_x": tx, "target_y": ty,
+        "t0": 0.0, "t_end": float(n_steps), "dt": 1.0, "stride": 50,
+    }
+
+
 def make_fields(cfg: dict):
     """Product field handles for a workload."""
     from .fields import Bathymetry, Current
-    b = Bathymetry.from_arrays(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
+    if "constant" in cfg["bathy"]:
+        b = Bathymetry.constant(cfg["bathy"]["constant"])
+    else:
+        b = Bathymetry.from_arrays(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
     c = Current.from_arrays(cfg["current"]["x"], cfg["current"]["y"], cfg["current"]["u"],
                             cfg["current"]["v"])
     return b, c

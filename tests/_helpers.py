@@ -6,7 +6,10 @@ import numpy as np
 
 
 def oracle_fields(O, cfg):
-    b = O.Bathymetry.grid(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
+    if "constant" in cfg["bathy"]:
+        b = O.Bathymetry.constant(cfg["bathy"]["constant"])
+    else:
+        b = O.Bathymetry.grid(cfg["bathy"]["x"], cfg["bathy"]["y"], cfg["bathy"]["depth"])
     c = O.Current.grid(cfg["current"]["x"], cfg["current"]["y"], cfg["current"]["u"],
                        cfg["current"]["v"])
     return b, c

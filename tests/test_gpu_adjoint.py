@@ -1,0 +1,383 @@
+"""The fp64-interpolation mode and its discrete adjoint on the GPU.
+
+  - mr_rhs_many and mr_trace_many in MR_PREC_FP64_INTERP against the oracle's
+    fp64-interp mode (same algorithm, f64 throughout): right-hand sides at
+    random states (grid nodes, cell lines, out of range, NaN) within 1e-12
+    relative, trajectories within 1e-9 m with identical row counts;
+  - checkpoints are bitwise the trajectory rows they stand for;
+  - mr_adjoint_many against central finite differences of the same forward:
+    one step (tight), many steps on small gridded depth and current (initial
+    states per ray, directional derivatives of the depth, u and v grids,
+    single grid values), a trajectory (all-rows) loss, rays that leave the
+    grid, and the eddy workload (512 x 512 grid, 16,384 rays, 2000 steps;
+    finite differences on 256 rays) with the loss sum |x(T) - target|^2;
+  - gradients with respect to the initial states do not depend on the
+    checkpoint stride (bitwise), and the torch.autograd.Function returns the
+    same gradients as the direct call.
+
+Finite differences of a 2000-step trace carry rounding noise of about
+1e-10 m in the endpoints, so a central difference with eps = 1e-6 m is good
+to about 1e-4 relative; the comparisons below use eps large enough for the
+noise to fall below the stated tolerance and small enough that no
+perturbed stage crosses a cell line (where the cell-difference slopes
+jump).
+"""
+import numpy as np
+import pytest
+
+from _helpers import oracle_fields, random_grid_case, random_states
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mr():
+    import mantaray_amd
+    mantaray_amd._capi.lib()
+    return mantaray_amd
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    return _t
+
+
+def _dev(torch, a):
+    return torch.tensor(np.asarray(a, dtype=np.float64), device="cuda:0")
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_fp64_rhs_matches_oracle(mr, oracle_mod, seed, math):
+    cfg = random_grid_case(seed)
+    b, c = mr.Bathymetry.from_arrays(**cfg["bathy"]), mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 4096)
+    ds = mr.rhs(b, c, s, math=math, precision="fp64")
+    ref, _ = oracle_mod.system_many(ob, oc, s, prec=oracle_mod.PREC_FP64_INTERP)
+    assert np.array_equal(np.isnan(ds), np.isnan(ref))
+    scale = np.nanmax(np.abs(ref), axis=0)
+    err = np.nanmax(np.abs(ds - ref) / scale)
+    assert err <= (1e-13 if math == "literal" else 1e-12), err
+
+
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_fp64_trace_matches_oracle(mr, oracle_mod, math):
+    cfg = random_grid_case(5)
+    b, c = mr.Bathymetry.from_arrays(**cfg["bathy"]), mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    rng = np.random.default_rng(5)
+    n = 512
+    th = rng.uniform(0, 2 * np.pi, n)
+    k = rng.uniform(0.01, 0.2, n)
+    args = (rng.uniform(-300.0, 300.0, n), rng.uniform(200.0, 900.0, n), k * np.cos(th),
+            k * np.sin(th), 0.0, 400.0, 2.0)
+    got = mr.trace_many(b, c, *args, math=math, precision="fp64")
+    ref = oracle_mod.trace_many(ob, oc, *args, prec=oracle_mod.PREC_FP64_INTERP)
+    assert np.array_equal(got["len"], ref["len"])
+    assert (ref["len"] < 201).sum() > 50          # many rays leave the grid
+    for key in ("x", "y", "kx", "ky"):
+        assert np.array_equal(np.isnan(got[key]), np.isnan(ref[key]))
+    assert np.nanmax(np.abs(got["x"] - ref["x"])) <= 1e-9
+    assert np.nanmax(np.abs(got["y"] - ref["y"])) <= 1e-9
+
+
+def test_fp64_trace_eddy_subset(mr, oracle_mod):
+    from mantaray_amd import workloads
+    cfg = workloads.eddy_adjoint(n_rays=512)
+    b, c = workloads.make_fields(cfg)
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    args = (cfg["x0"], cfg["y0"], cfg["kx0"], cfg["ky0"], cfg["t0"], cfg["t_end"], cfg["dt"])
+    got = mr.trace_many(b, c, *args, precision="fp64", trajectories=False)
+    ref = oracle_mod.trace_many(ob, oc, *args, prec=oracle_mod.PREC_FP64_INTERP,
+                                trajectories=False)
+    assert np.array_equal(got["len"], ref["len"])
+    d = np.abs(got["end"][:, :2] - ref["end"][:, :2]).max()
+    print(f"eddy fp64 endpoints: max |d(x, y)| {d:.3e} m")
+    assert d <= 1e-8
+
+
+def test_checkpoints_are_rows(mr, torch):
+    from mantaray_amd import adjoint, workloads
+    cfg = workloads.eddy_adjoint(n_rays=300, n_steps=230)
+    b, c = workloads.make_fields(cfg)
+    ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
+    r = adjoint.trace_fp64(b, c, *ic, 0.0, 230.0, 1.0, stride=50, trajectories=True)
+    torch.cuda.synchronize()
+    ck = r["ckpt"].cpu().numpy()
+    assert ck.shape == (5, 4, 300)
+    rows = np.stack([r[k].cpu().numpy() for k in ("x", "y", "kx", "ky")], axis=1)
+    for i in range(5):
+        assert np.array_equal(ck[i], rows[50 * i])
+    full = mr.trace_many(b, c, *cfg_args(cfg, 230.0), precision="fp64")
+    assert np.array_equal(full["x"], r["x"].cpu().numpy())
+    assert np.array_equal(full["end"], r["end"].cpu().numpy())
+
+
+def cfg_args(cfg, t_end):
+    return (cfg["x0"], cfg["y0"], cfg["kx0"], cfg["ky0"], cfg["t0"], t_end, cfg["dt"])
+
+
+# ---------------------------------------------------------------------------
+# finite differences
+
+
+def _fd_ic(adjoint, b, c, ic, comp, eps, t_end, dt, tx, ty, **kw):
+    """Per-ray central difference of L_r = (x_r - tx)^2 + (y_r - ty)^2."""
+    plus = [t.clone() for t in ic]
+    minus = [t.clone() for t in ic]
+    plus[comp] += eps
+    minus[comp] -= eps
+    ep = adjoint.trace_fp64(b, c, *plus, 0.0, t_end, dt, **kw)["end"].cpu().numpy()
+    em = adjoint.trace_fp64(b, c, *minus, 0.0, t_end, dt, **kw)["end"].cpu().numpy()
+    dl = ((ep[:, 0] - em[:, 0]) * (ep[:, 0] + em[:, 0] - 2 * tx)
+          + (ep[:, 1] - em[:, 1]) * (ep[:, 1] + em[:, 1] - 2 * ty))
+    return dl / (2 * eps)
+
+
+def _fd_values(adjoint, b, c, ic, t_end, dt, tx, ty, eps, vals, pert):
+    """Central difference of sum_r L_r along a direction in the value arrays
+    (vals: name -> values; pert: name -> direction tensor)."""
+    vals = {k: vals.get(k) for k in ("depth", "u", "v")}
+    plus = {k: (None if a is None else a.clone()) for k, a in vals.items()}
+    minus = {k: (None if a is None else a.clone()) for k, a in vals.items()}
+    for k, d in pert.items():
+        plus[k] = plus[k] + eps * d
+        minus[k] = minus[k] - eps * d
+    ep = adjoint.trace_fp64(b, c, *ic, 0.0, t_end, dt, **plus)["end"].cpu().numpy()
+    em = adjoint.trace_fp64(b, c, *ic, 0.0, t_end, dt, **minus)["end"].cpu().numpy()
+    ok = ~np.isnan(ep[:, 0]) & ~np.isnan(em[:, 0])
+    dl = ((ep[:, 0] - em[:, 0]) * (ep[:, 0] + em[:, 0] - 2 * tx)
+          + (ep[:, 1] - em[:, 1]) * (ep[:, 1] + em[:, 1] - 2 * ty))
+    return float(np.sum(dl[ok])) / (2 * eps)
+
+
+def _g_end(end, tx, ty, torch):
+    e = end.cpu().numpy()
+    g = np.zeros_like(e)
+    g[:, 0] = 2 * (e[:, 0] - tx)
+    g[:, 1] = 2 * (e[:, 1] - ty)
+    g[np.isnan(g)] = 0.0
+    return torch.tensor(g, device=end.device)
+
+
+def _report(tag, g, fd):
+    """Print the distribution of per-ray relative errors."""
+    r = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+    q = np.quantile(r, [0.5, 0.9, 0.99, 1.0])
+    print(f"{tag}: per-ray rel err median {q[0]:.1e} p90 {q[1]:.1e} p99 {q[2]:.1e} "
+          f"max {q[3]:.1e}; normwise {_rel(g, fd):.1e}")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _small_case(mr, torch, n=256, seed=7):
+    cfg = random_grid_case(seed, nx=41, ny=37, dx=40.0, dy=45.0, x0=-800.0, y0=-700.0)
+    b, c = mr.Bathymetry.from_arrays(**cfg["bathy"]), mr.Current.from_arrays(**cfg["current"])
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, n)
+    k = rng.uniform(0.03, 0.15, n)
+    x0 = rng.uniform(-500.0, 500.0, n)
+    y0 = rng.uniform(-400.0, 500.0, n)
+    ic = [_dev(torch, a) for a in (x0, y0, k * np.cos(th), k * np.sin(th))]
+    tx = x0 + rng.uniform(-200, 200, n)
+    ty = y0 + rng.uniform(-200, 200, n)
+    vals = {"depth": _dev(torch, cfg["bathy"]["depth"].ravel()),
+            "u": _dev(torch, cfg["current"]["u"].ravel()),
+            "v": _dev(torch, cfg["current"]["v"].ravel())}
+    return b, c, ic, tx, ty, vals
+
+
+def test_adjoint_one_step(mr, torch):
+    """One RK4 step: the adjoint of the four stage products against central
+    differences (no accumulated rounding, so the check is tight)."""
+    from mantaray_amd import adjoint
+    b, c, ic, tx, ty, vals = _small_case(mr, torch)
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, 1.0, 1.0, stride=1, **vals)
+    ge = _g_end(fwd["end"], tx, ty, torch)
+    g = adjoint.adjoint(b, c, fwd, 0.0, 1.0, 1.0, 1, g_end=ge, **vals)
+    torch.cuda.synchronize()
+    for comp, name, eps in ((0, "x0", 1e-4), (1, "y0", 1e-4), (2, "kx0", 1e-8), (3, "ky0", 1e-8)):
+        fd = _fd_ic(adjoint, b, c, ic, comp, eps, 1.0, 1.0, tx, ty, **vals)
+        err = _rel(g[name].cpu().numpy(), fd)
+        print(f"one step d/d{name}: rel err {err:.2e}")
+        assert err <= 1e-6, (name, err)
+
+
+@pytest.mark.parametrize("stride", [1, 16])
+def test_adjoint_small_grid(mr, torch, stride):
+    """120 steps over rough gridded depth and current (rays leave the grid):
+    initial-state gradients per ray, and depth / u / v directional
+    derivatives, against central differences."""
+    from mantaray_amd import adjoint
+    b, c, ic, tx, ty, vals = _small_case(mr, torch)
+    T = 120.0
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=stride, **vals)
+    ln = fwd["len"].cpu().numpy()
+    assert (ln < 121).sum() > 10                     # some rays leave the grid
+    ge = _g_end(fwd["end"], tx, ty, torch)
+    g = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, stride, g_end=ge, **vals)
+    torch.cuda.synchronize()
+    dead = ln < 121
+    for comp, name, eps in ((0, "x0", 1e-5), (1, "y0", 1e-5), (2, "kx0", 1e-9), (3, "ky0", 1e-9)):
+        fd = _fd_ic(adjoint, b, c, ic, comp, eps, T, 1.0, tx, ty, **vals)
+        gg = g[name].cpu().numpy()
+        assert np.all(gg[dead] == 0.0)               # a NaN last row passes nothing
+        live = ~dead & np.isfinite(fd)
+        assert live.sum() > 150
+        _report(f"stride {stride} d/d{name}", gg[live], fd[live])
+        err = _rel(gg[live], fd[live])
+        print(f"stride {stride} d/d{name}: rel err {err:.2e}")
+        assert err <= 1e-5, (name, err)
+    rng = np.random.default_rng(3)
+    for name in ("depth", "u", "v"):
+        d = rng.normal(size=vals[name].numel())
+        d /= np.linalg.norm(d)
+        dt_ = _dev(torch, d)
+        fd = _fd_values(adjoint, b, c, ic, T, 1.0, tx, ty, 1e-6, vals, {name: dt_})
+        an = float(np.dot(g[name].cpu().numpy(), d))
+        err = abs(an - fd) / abs(fd)
+        print(f"stride {stride} directional d/d{name}: adjoint {an:.9e} fd {fd:.9e} rel {err:.2e}")
+        assert err <= 1e-5, (name, err)
+
+
+def test_adjoint_single_values(mr, torch):
+    """Gradient entries of single u and depth values near many rays."""
+    from mantaray_amd import adjoint
+    b, c, ic, tx, ty, vals = _small_case(mr, torch, n=1024)
+    T = 60.0
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=10, **vals)
+    ge = _g_end(fwd["end"], tx, ty, torch)
+    g = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, 10, g_end=ge, **vals)
+    torch.cuda.synchronize()
+    for name in ("depth", "u", "v"):
+        ga = g[name].cpu().numpy()
+        for idx in np.argsort(-np.abs(ga))[:3]:
+            d = torch.zeros_like(vals[name])
+            d[idx] = 1.0
+            fd = _fd_values(adjoint, b, c, ic, T, 1.0, tx, ty, 1e-6, vals, {name: d})
+            err = abs(ga[idx] - fd) / abs(fd)
+            print(f"d/d{name}[{idx}]: adjoint {ga[idx]:.9e} fd {fd:.9e} rel {err:.2e}")
+            assert err <= 1e-5, (name, idx, err)
+
+
+def test_adjoint_rows_loss(mr, torch):
+    """A loss on every row: L = sum_rows w_s . (x_s, y_s, kx_s, ky_s)."""
+    from mantaray_amd import adjoint
+    b, c, ic, tx, ty, vals = _small_case(mr, torch)
+    T = 80.0
+    n = 80
+    rng = np.random.default_rng(11)
+    w = {k: rng.normal(size=(n + 1, 256)) * s for k, s in
+         (("x", 1.0), ("y", 1.0), ("kx", 1e3), ("ky", 1e3))}
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=7, trajectories=True, **vals)
+    rows = {k: _dev(torch, w[k]) for k in w}
+    g = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, 7, g_rows=rows, **vals)
+    torch.cuda.synchronize()
+
+    def loss_rows(icx):
+        r = adjoint.trace_fp64(b, c, *icx, 0.0, T, 1.0, trajectories=True, **vals)
+        tot = np.zeros(256)
+        for k in w:
+            a = r[k].cpu().numpy()
+            tot += np.nansum(np.where(np.isnan(a), 0.0, a * w[k]), axis=0)
+        return tot
+
+    for comp, name, eps in ((0, "x0", 1e-5), (3, "ky0", 1e-9)):
+        plus = [t.clone() for t in ic]
+        minus = [t.clone() for t in ic]
+        plus[comp] += eps
+        minus[comp] -= eps
+        fd = (loss_rows(plus) - loss_rows(minus)) / (2 * eps)
+        err = _rel(g[name].cpu().numpy(), fd)
+        print(f"rows loss d/d{name}: rel err {err:.2e}")
+        assert err <= 1e-5, (name, err)
+
+
+def test_stride_independence_and_autograd(mr, torch):
+    from mantaray_amd import adjoint, workloads
+    cfg = workloads.eddy_adjoint(n_rays=700, n_steps=300)
+    b, c = workloads.make_fields(cfg)
+    ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
+    u = _dev(torch, cfg["current"]["u"].ravel())
+    v = _dev(torch, cfg["current"]["v"].ravel())
+    tx, ty = cfg["target_x"], cfg["target_y"]
+    res = {}
+    for stride in (1, 7, 50, 300, 1000):
+        fwd = adjoint.trace_fp64(b, c, *ic, 0.0, 300.0, 1.0, stride=stride, u=u, v=v)
+        ge = _g_end(fwd["end"], tx, ty, torch)
+        res[stride] = adjoint.adjoint(b, c, fwd, 0.0, 300.0, 1.0, stride, g_end=ge, u=u, v=v)
+    for stride in (7, 50, 300, 1000):
+        for k in ("x0", "y0", "kx0", "ky0"):
+            assert torch.equal(res[stride][k], res[1][k]), (stride, k)
+        for k in ("u", "v"):
+            d = (res[stride][k] - res[1][k]).abs().max().item()
+            assert d <= 1e-9 * res[1][k].abs().max().item(), (stride, k, d)
+    # the autograd.Function
+    leaves = [t.clone().requires_grad_(True) for t in ic]
+    ul, vl = u.clone().requires_grad_(True), v.clone().requires_grad_(True)
+    end, ln = adjoint.trace_differentiable(b, c, *leaves, 0.0, 300.0, 1.0, u=ul, v=vl,
+                                           stride=50)
+    txt, tyt = _dev(torch, tx), _dev(torch, ty)
+    loss = ((end[:, 0] - txt) ** 2 + (end[:, 1] - tyt) ** 2).sum()
+    loss.backward()
+    for t, k in zip(leaves, ("x0", "y0", "kx0", "ky0")):
+        assert torch.equal(t.grad, res[50][k]), k
+    for t, k in ((ul, "u"), (vl, "v")):
+        d = (t.grad - res[50][k]).abs().max().item()
+        assert d <= 1e-9 * res[50][k].abs().max().item(), k
+
+
+def test_adjoint_eddy_workload(mr, torch):
+    """The eddy workload (512 x 512, 16,384 rays, 2000 steps, checkpoint
+    stride 50): gradients of sum_r |x_r(T) - target_r|^2 against central
+    differences on a 256-ray subset, eps = 1e-6 (the workload's step) and
+    eps = 1e-4 (where the endpoints' rounding noise is negligible)."""
+    from mantaray_amd import adjoint, workloads
+    cfg = workloads.eddy_adjoint()
+    b, c = workloads.make_fields(cfg)
+    ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
+    u = _dev(torch, cfg["current"]["u"].ravel())
+    v = _dev(torch, cfg["current"]["v"].ravel())
+    tx, ty = cfg["target_x"], cfg["target_y"]
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, cfg["t_end"], 1.0, stride=50, u=u, v=v)
+    ge = _g_end(fwd["end"], tx, ty, torch)
+    g = adjoint.adjoint(b, c, fwd, 0.0, cfg["t_end"], 1.0, 50, g_end=ge, u=u, v=v)
+    torch.cuda.synchronize()
+    assert int((fwd["len"] < 2001).sum()) == 0
+    sub = slice(0, 256)
+    ics = [t[sub].clone() for t in ic]
+    tsx, tsy = tx[sub], ty[sub]
+    # d x(T) / d k0 is about 1e6 m per rad/m, so the wavenumber steps are 1e-6
+    # of the position steps: both move the endpoints by about eps
+    for comp, name, scale in ((0, "x0", 1.0), (1, "y0", 1.0), (2, "kx0", 1e-6),
+                              (3, "ky0", 1e-6)):
+        for eps in (1e-6, 1e-5, 1e-4):
+            fd = _fd_ic(adjoint, b, c, ics, comp, eps * scale, cfg["t_end"], 1.0, tsx, tsy,
+                        u=u, v=v)
+            gg = g[name][sub].cpu().numpy()
+            _report(f"eddy d/d{name} eps {eps * scale:.0e}", gg, fd)
+            err = _rel(gg, fd)
+            if eps == 1e-6:
+                assert err <= 1e-3, (name, eps, err)
+    # grid values: directional derivatives over the 256-ray subset's loss
+    fws = adjoint.trace_fp64(b, c, *ics, 0.0, cfg["t_end"], 1.0, stride=50, u=u, v=v)
+    gs = adjoint.adjoint(b, c, fws, 0.0, cfg["t_end"], 1.0, 50,
+                         g_end=_g_end(fws["end"], tsx, tsy, torch), u=u, v=v)
+    rng = np.random.default_rng(4)
+    for name in ("u", "v"):
+        d = rng.normal(size=u.numel())
+        d /= np.linalg.norm(d)
+        an = float(np.dot(gs[name].cpu().numpy(), d))
+        for eps in (1e-6, 1e-4):
+            fd = _fd_values(adjoint, b, c, ics, cfg["t_end"], 1.0, tsx, tsy, eps,
+                            {"u": u, "v": v}, {name: _dev(torch, d)})
+            err = abs(an - fd) / abs(fd)
+            print(f"eddy directional d/d{name} eps {eps:.0e}: adjoint {an:.9e} fd {fd:.9e} "
+                  f"rel {err:.2e}")
+            if eps == 1e-6:
+                assert err <= 1e-3, (name, eps, err)
