@@ -9,6 +9,13 @@ trajectories written to HBM.  Initial conditions, field tables and output
 buffers are resident on the GPU before the timed region; the timed region is
 K back-to-back launches of the trace kernel.
 
+The second half of the metric, the adjoint's cost, is measured on the
+eddy-adjoint case (512 x 512 grid at 250 m, constant depth 50 m, 20 Gaussian
+eddies, 16,384 rays, 2000 steps, checkpoint stride 50, loss on the ray
+endpoints): the fp64-interpolation forward alone, and that forward with its
+checkpoints followed by the reverse sweep (gradients with respect to the
+initial states and the u, v grids), each timed with events on its stream.
+
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
 its own event-timed duration; HBM bytes per launch from the rocprofv3
@@ -45,6 +52,7 @@ def parse():
     p.add_argument("--math", default="fast", choices=["fast", "literal"])
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--cpu-rays", type=int, default=16384,
                    help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
@@ -97,11 +105,83 @@ def cpu_baseline(cfg, n_rays, runs=3, single_rays=256):
             "host": {"cpu": model, "nproc": ncpu}}
 
 
+def _event_ms(fn, stream, reps):
+    """Median duration of fn() between events on `stream` over reps runs."""
+    import numpy as np
+    import torch
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def adjoint_cost(dev, reps=5):
+    """fwd+bwd / fwd on the eddy-adjoint workload (see the module docstring)."""
+    import torch
+
+    from mantaray_amd import adjoint, core, workloads
+    cfg = workloads.eddy_adjoint()
+    b, c = workloads.make_fields(cfg)
+    b.upload(0)
+    c.upload(0)
+    T, dt, stride = cfg["t_end"], cfg["dt"], cfg["stride"]
+    ic = [torch.tensor(cfg[k], dtype=torch.float64, device=dev) for k in ("x0", "y0", "kx0", "ky0")]
+    u = torch.tensor(cfg["current"]["u"].ravel(), device=dev)
+    v = torch.tensor(cfg["current"]["v"].ravel(), device=dev)
+    tx = torch.tensor(cfg["target_x"], device=dev)
+    ty = torch.tensor(cfg["target_y"], device=dev)
+    nr = tx.numel()
+    stream = torch.cuda.current_stream(dev)
+    work = torch.empty(adjoint.workspace_bytes(nr, stride), dtype=torch.uint8, device=dev)
+    state = {}
+
+    def fwd_plain():
+        adjoint.trace_fp64(b, c, *ic, 0.0, T, dt, u=u, v=v, stream=stream)
+
+    def fwd_bwd():
+        f = adjoint.trace_fp64(b, c, *ic, 0.0, T, dt, u=u, v=v, stride=stride, stream=stream)
+        e = f["end"]
+        g = torch.zeros_like(e)
+        g[:, 0] = 2.0 * (e[:, 0] - tx)
+        g[:, 1] = 2.0 * (e[:, 1] - ty)
+        state["g"] = adjoint.adjoint(b, c, f, 0.0, T, dt, stride, g_end=g, u=u, v=v,
+                                     stream=stream, work=work)
+        state["f"] = f
+
+    def fwd_ref():
+        core.trace_many_device(b, c, *ic, 0.0, T, dt, trajectories=False, stream=stream)
+
+    for fn in (fwd_plain, fwd_bwd, fwd_ref):
+        fn()
+    torch.cuda.synchronize()
+    t_f = _event_ms(fwd_plain, stream, reps)
+    t_fb = _event_ms(fwd_bwd, stream, reps)
+    t_ref = _event_ms(fwd_ref, stream, reps)
+    f = state["f"]
+    ray_steps = int((f["len"].to(torch.int64) - 1).sum().item())
+    return {"workload": "eddy_adjoint", "grid": "512x512 @ 250 m", "rays": nr,
+            "rk4_steps": int(T / dt), "checkpoint_stride": stride,
+            "loss": "sum_r |(x, y)_r(T) - target_r|^2",
+            "gradients": "initial (x, y, kx, ky) of every ray; every u and v grid value",
+            "fwd_ms": t_f, "fwd_bwd_ms": t_fb, "ratio": t_fb / t_f,
+            "fwd_reference_precision_ms": t_ref, "ratio_vs_reference_precision_fwd": t_fb / t_ref,
+            "ray_steps": ray_steps,
+            "checkpoint_bytes": int(f["ckpt"].numel() * 8),
+            "workspace_bytes": int(work.numel()),
+            "note": "fwd = the fp64-interpolation forward the adjoint differentiates, endpoints "
+                    "only; fwd+bwd = that forward with checkpoints, then the reverse sweep"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(rays shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(rays shard across GPUs with no exchange step)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -155,7 +235,7 @@ def main():
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     res = {
-        "metric": "FP64 RK ray-steps/s on 1 MI355X vs host CPU",
+        "metric": "FP64 RK ray-steps/s on 1 MI355X vs host CPU; adjoint fwd+bwd / fwd cost",
         "value": value,
         "unit": "ray-steps/s",
         "n_gpus": 1,
@@ -179,6 +259,8 @@ def main():
                      "hbm_GBps": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9,
                      "hbm_peak_GBps": 8000.0},
     }
+    if not args.no_adjoint:
+        res["adjoint"] = adjoint_cost(dev)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_rays, nr))
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]

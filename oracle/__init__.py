@@ -96,6 +96,10 @@ def lib():
                                      ctypes.c_int, ctypes.c_int64, _f64p, _f64p, _i32p]
         L.or_system_many.restype = None
         L.or_max_threads.restype = ctypes.c_int
+        L.or_fd_end_ld.argtypes = [ctypes.POINTER(_Bathy), ctypes.POINTER(_Current),
+                                   ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, _f64p, _f64p, _i32p, _i32p]
         L.or_bathy_nearest.argtypes = [ctypes.POINTER(_Bathy), ctypes.c_float, ctypes.c_int, _f32p]
         L.or_current_nearest.argtypes = [ctypes.POINTER(_Current), ctypes.c_double, ctypes.c_int,
                                          _f64p]
@@ -308,6 +312,27 @@ def trace_many(b, c, x0, y0, kx0, ky0, t0, t_end, dt, prec=PREC_REFERENCE,
     for name, o in zip(("x", "y", "kx", "ky"), outs):
         res[name] = o
     return res
+
+
+def fd_end_ld(b, c, s0, t0, t_end, dt, eps=0.0, ds0=None, ddepth=None, du=None, dv=None,
+              nthreads=0):
+    """Long-double traces from s0 +- eps ds0 with values +- eps d (fp64-interp
+    algorithm): returns diff = end(+) - end(-), mid = (end(+) + end(-))/2
+    (n_rays, 4) and the two row counts."""
+    s0 = np.ascontiguousarray(s0, dtype=np.float64).reshape(-1, 4)
+    nr = s0.shape[0]
+    arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+            for a in (ds0, ddepth, du, dv)]
+    diff = np.empty((nr, 4))
+    mid = np.empty((nr, 4))
+    lp = np.empty(nr, dtype=np.int32)
+    lm = np.empty(nr, dtype=np.int32)
+    rc = lib().or_fd_end_ld(ctypes.byref(b.s), ctypes.byref(c.s), nr, _p(s0, _f64p),
+                            *[_p(a, _f64p) for a in arrs], eps, t0, t_end, dt, nthreads,
+                            _p(diff, _f64p), _p(mid, _f64p), _p(lp, _i32p), _p(lm, _i32p))
+    if rc != 0:
+        raise ValueError("invalid step count (the reference panics)")
+    return diff, mid, lp, lm
 
 
 def max_threads():

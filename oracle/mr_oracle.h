@@ -104,6 +104,18 @@ void or_system_many(const or_bathy *b, const or_current *c, int prec, int64_t n,
 
 int or_max_threads(void);
 
+/* mr_oracle_ld.c: the fp64-interpolation trace in long double, as the
+ * finite-difference truth of the adjoint checks.  Each ray is traced from
+ * s0 + eps ds0 and from s0 - eps ds0 (s0, ds0: 4 per ray; ds0 may be NULL)
+ * with the gridded values + eps d and - eps d (ddepth, du, dv may be NULL);
+ * diff = end(+) - end(-) and mid = (end(+) + end(-))/2, each formed in long
+ * double, 4 per ray; len_plus / len_minus the row counts (may be NULL).
+ * Returns 0, or -1 on a bad step count. */
+int or_fd_end_ld(const or_bathy *b, const or_current *c, int64_t n_rays, const double *s0,
+                 const double *ds0, const double *ddepth, const double *du, const double *dv,
+                 double eps, double t0, double t_end, double dt, int nthreads, double *diff,
+                 double *mid, int32_t *len_plus, int32_t *len_minus);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,12 +15,12 @@
     checkpoint stride (bitwise), and the torch.autograd.Function returns the
     same gradients as the direct call.
 
-Finite differences of a 2000-step trace carry rounding noise of about
-1e-10 m in the endpoints, so a central difference with eps = 1e-6 m is good
-to about 1e-4 relative; the comparisons below use eps large enough for the
-noise to fall below the stated tolerance and small enough that no
-perturbed stage crosses a cell line (where the cell-difference slopes
-jump).
+Finite differences of a 2000-step f64 trace carry its rounding noise (about
+1e-10 m in the endpoints), so with eps = 1e-6 m they are good to about 1e-4
+relative only; the eddy check therefore takes its truth from the same
+algorithm traced in long double (the oracle's or_fd_end_ld, 2^11 times less
+noise).  Steps stay small enough that no perturbed stage point crosses a
+cell line, where the cell-difference slopes jump.
 """
 import numpy as np
 import pytest
@@ -133,7 +133,8 @@ def _fd_ic(adjoint, b, c, ic, comp, eps, t_end, dt, tx, ty, **kw):
     em = adjoint.trace_fp64(b, c, *minus, 0.0, t_end, dt, **kw)["end"].cpu().numpy()
     dl = ((ep[:, 0] - em[:, 0]) * (ep[:, 0] + em[:, 0] - 2 * tx)
           + (ep[:, 1] - em[:, 1]) * (ep[:, 1] + em[:, 1] - 2 * ty))
-    return dl / (2 * eps)
+    step = (plus[comp] - minus[comp]).cpu().numpy()      # the step as represented
+    return dl / step
 
 
 def _fd_values(adjoint, b, c, ic, t_end, dt, tx, ty, eps, vals, pert):
@@ -332,52 +333,74 @@ def test_stride_independence_and_autograd(mr, torch):
         assert d <= 1e-9 * res[50][k].abs().max().item(), k
 
 
-def test_adjoint_eddy_workload(mr, torch):
+def _ld_loss_fd(oracle_mod, ob, oc, s0, T, tx, ty, eps, **dirs):
+    """Central difference of sum_r L_r from the long-double traces."""
+    diff, mid, _, _ = oracle_mod.fd_end_ld(ob, oc, s0, 0.0, T, 1.0, eps=eps, **dirs)
+    return (diff[:, 0] * 2 * (mid[:, 0] - tx) + diff[:, 1] * 2 * (mid[:, 1] - ty)) / (2 * eps)
+
+
+def test_adjoint_eddy_workload(mr, oracle_mod, torch):
     """The eddy workload (512 x 512, 16,384 rays, 2000 steps, checkpoint
-    stride 50): gradients of sum_r |x_r(T) - target_r|^2 against central
-    differences on a 256-ray subset, eps = 1e-6 (the workload's step) and
-    eps = 1e-4 (where the endpoints' rounding noise is negligible)."""
+    stride 50): gradients of sum_r |x_r(T) - target_r|^2 on a 256-ray subset
+    against central differences with eps = 1e-6 (wavenumbers: 1e-12, which
+    moves the endpoints as far).  Truth: the same algorithm traced in long
+    double (or_fd_end_ld), within 1e-6 normwise; the f64 forward's own
+    central differences agree to their rounding noise (1e-3)."""
     from mantaray_amd import adjoint, workloads
     cfg = workloads.eddy_adjoint()
     b, c = workloads.make_fields(cfg)
+    ob, oc = oracle_fields(oracle_mod, cfg)
     ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
     u = _dev(torch, cfg["current"]["u"].ravel())
     v = _dev(torch, cfg["current"]["v"].ravel())
     tx, ty = cfg["target_x"], cfg["target_y"]
-    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, cfg["t_end"], 1.0, stride=50, u=u, v=v)
+    T = cfg["t_end"]
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=50, u=u, v=v)
     ge = _g_end(fwd["end"], tx, ty, torch)
-    g = adjoint.adjoint(b, c, fwd, 0.0, cfg["t_end"], 1.0, 50, g_end=ge, u=u, v=v)
+    g = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, 50, g_end=ge, u=u, v=v)
     torch.cuda.synchronize()
     assert int((fwd["len"] < 2001).sum()) == 0
     sub = slice(0, 256)
+    s0 = np.stack([cfg[k][sub] for k in ("x0", "y0", "kx0", "ky0")], 1)
     ics = [t[sub].clone() for t in ic]
     tsx, tsy = tx[sub], ty[sub]
-    # d x(T) / d k0 is about 1e6 m per rad/m, so the wavenumber steps are 1e-6
-    # of the position steps: both move the endpoints by about eps
     for comp, name, scale in ((0, "x0", 1.0), (1, "y0", 1.0), (2, "kx0", 1e-6),
                               (3, "ky0", 1e-6)):
-        for eps in (1e-6, 1e-5, 1e-4):
-            fd = _fd_ic(adjoint, b, c, ics, comp, eps * scale, cfg["t_end"], 1.0, tsx, tsy,
-                        u=u, v=v)
-            gg = g[name][sub].cpu().numpy()
-            _report(f"eddy d/d{name} eps {eps * scale:.0e}", gg, fd)
-            err = _rel(gg, fd)
-            if eps == 1e-6:
-                assert err <= 1e-3, (name, eps, err)
-    # grid values: directional derivatives over the 256-ray subset's loss
-    fws = adjoint.trace_fp64(b, c, *ics, 0.0, cfg["t_end"], 1.0, stride=50, u=u, v=v)
-    gs = adjoint.adjoint(b, c, fws, 0.0, cfg["t_end"], 1.0, 50,
-                         g_end=_g_end(fws["end"], tsx, tsy, torch), u=u, v=v)
+        gg = g[name][sub].cpu().numpy()
+        ds0 = np.zeros_like(s0)
+        ds0[:, comp] = 1.0
+        fd = _ld_loss_fd(oracle_mod, ob, oc, s0, T, tsx, tsy, 1e-6 * scale, ds0=ds0)
+        _report(f"eddy d/d{name} (long-double FD)", gg, fd)
+        err = _rel(gg, fd)
+        assert err <= 1e-6, (name, err)
+        fd64 = _fd_ic(adjoint, b, c, ics, comp, 1e-6 * scale, T, 1.0, tsx, tsy, u=u, v=v)
+        _report(f"eddy d/d{name} (f64 FD)", gg, fd64)
+        assert _rel(gg, fd64) <= 1e-3, name
+    # grid values: directional derivatives of the subset's loss
+    fws = adjoint.trace_fp64(b, c, *ics, 0.0, T, 1.0, stride=50, u=u, v=v)
+    gs = adjoint.adjoint(b, c, fws, 0.0, T, 1.0, 50, g_end=_g_end(fws["end"], tsx, tsy, torch),
+                         u=u, v=v)
     rng = np.random.default_rng(4)
     for name in ("u", "v"):
         d = rng.normal(size=u.numel())
         d /= np.linalg.norm(d)
         an = float(np.dot(gs[name].cpu().numpy(), d))
-        for eps in (1e-6, 1e-4):
-            fd = _fd_values(adjoint, b, c, ics, cfg["t_end"], 1.0, tsx, tsy, eps,
-                            {"u": u, "v": v}, {name: _dev(torch, d)})
-            err = abs(an - fd) / abs(fd)
-            print(f"eddy directional d/d{name} eps {eps:.0e}: adjoint {an:.9e} fd {fd:.9e} "
-                  f"rel {err:.2e}")
-            if eps == 1e-6:
-                assert err <= 1e-3, (name, eps, err)
+        fd = float(np.sum(_ld_loss_fd(oracle_mod, ob, oc, s0, T, tsx, tsy, 1e-6,
+                                      **{"d" + name: d})))
+        err = abs(an - fd) / abs(fd)
+        print(f"eddy directional d/d{name}: adjoint {an:.12e} long-double fd {fd:.12e} "
+              f"rel {err:.2e}")
+        assert err <= 1e-6, (name, err)
+        fd64 = _fd_values(adjoint, b, c, ics, T, 1.0, tsx, tsy, 1e-6, {"u": u, "v": v},
+                          {name: _dev(torch, d)})
+        print(f"eddy directional d/d{name}: f64 fd {fd64:.12e} rel {abs(an - fd64) / abs(fd64):.2e}")
+        assert abs(an - fd64) <= 1e-3 * abs(fd64)
+    # single values: the largest entries of the u gradient
+    gu = gs["u"].cpu().numpy()
+    for idx in np.argsort(-np.abs(gu))[:2]:
+        d = np.zeros(u.numel())
+        d[idx] = 1.0
+        fd = float(np.sum(_ld_loss_fd(oracle_mod, ob, oc, s0, T, tsx, tsy, 1e-6, du=d)))
+        err = abs(gu[idx] - fd) / abs(fd)
+        print(f"eddy d/du[{idx}]: adjoint {gu[idx]:.12e} long-double fd {fd:.12e} rel {err:.2e}")
+        assert err <= 1e-6, (idx, err)
