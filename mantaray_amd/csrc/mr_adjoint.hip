@@ -27,21 +27,28 @@
 namespace mr {
 
 // One RK4 step in fp64 mode from s: the first three stage derivatives k and
-// the new state o.  The arithmetic is that of trace_kernel's step.
+// the new state o.  The arithmetic is that of trace_kernel's step, and so is
+// the cell prefetch: each evaluation first starts loading, into sn, the cells
+// the next evaluation will most likely need (its point guessed from the
+// latest derivative (px, py)), then evaluates with the cells in hand.
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs &BP,
                                            const Grid64View &C, const Grid64Ptrs &CP,
-                                           const Fields64 &F, const FastConsts &K, double dt,
-                                           double half, double sixth, const double s[4],
-                                           double k[3][4], double o[4])
+                                           const Fields64 &F, const FastConsts &K, Slots64 &sn,
+                                           double &px, double &py, double dt, double half,
+                                           double sixth, const double s[4], double k[3][4],
+                                           double o[4])
 {
     double sx = s[0], sy = s[1], skx = s[2], sky = s[3];
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const double hs = st < 2 ? half : dt;
+        Slots64 sq = sn;
+        prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, s[0] + px * hs, s[1] + py * hs);
         double dx, dy, dkx, dky;
-        rhs64<BK, CK, MATH>(B, BP, C, CP, F, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        rhs64<BK, CK, MATH>(B, BP, C, CP, F, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        px = dx; py = dy;
         if (st < 3) {
             k[st][0] = dx; k[st][1] = dy; k[st][2] = dkx; k[st][3] = dky;
         }
@@ -89,6 +96,9 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
     const Grid64View C = load_view<Grid64View>(a.C);
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     const double half = a.half, dt = a.dt, sixth = a.sixth;
+    Slots64 sn;                        // cells for the next evaluation
+    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, Slots64{}, sn, s[0], s[1]);
+    double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
     bool alive = true;
     int64_t len = 1;
     int64_t s_i = 1;
@@ -107,7 +117,8 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
         double o[4] = {MR_NAN, MR_NAN, MR_NAN, MR_NAN};
         if (alive) {
             double k[3][4], n[4];
-            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, dt, half, sixth, s, k, n);
+            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half, sixth, s,
+                                     k, n);
             if (a.solout == 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
                 alive = false;         // the ray ended on the row before
             } else {
@@ -160,9 +171,10 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
 template <int BK, int CK>
 __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid64Ptrs &BP,
                                                  const Grid64View &C, const Grid64Ptrs &CP,
-                                                 const Fields64 &F, const Grads64 &Gr,
-                                                 Accs64 &acc, const FastConsts &K, double dt,
-                                                 double half, double sixth, const double s[4],
+                                                 const Fields64 &F, Slots64 &sn,
+                                                 const Grads64 &Gr, Accs64 &acc,
+                                                 const FastConsts &K, double dt, double half,
+                                                 double sixth, const double s[4],
                                                  const double k[3][4], double lam[4])
 {
     double kb[4][4];
@@ -172,19 +184,33 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
         kb[0][q] = w; kb[3][q] = w;
         kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
     }
+    double Y[4][2];                    // (x, y) of the four stage points
+    Y[0][0] = s[0]; Y[0][1] = s[1];
+#pragma unroll
+    for (int st = 1; st < 4; ++st) {
+        const double hs = st < 3 ? half : dt;
+        Y[st][0] = s[0] + k[st - 1][0] * hs;
+        Y[st][1] = s[1] + k[st - 1][1] * hs;
+    }
 #pragma unroll
     for (int st = 3; st >= 0; --st) {
-        double Y[4];
+        double Yk[2];
         if (st == 0) {
-            Y[0] = s[0]; Y[1] = s[1]; Y[2] = s[2]; Y[3] = s[3];
+            Yk[0] = s[2]; Yk[1] = s[3];
         } else {
             const double hs = st < 3 ? half : dt;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Y[q] = s[q] + k[st - 1][q] * hs;
+            Yk[0] = s[2] + k[st - 1][2] * hs;
+            Yk[1] = s[3] + k[st - 1][3] * hs;
         }
+        // the next evaluation is at the previous stage point, known exactly;
+        // after stage 1 it is the previous step's last stage point, next to
+        // this step's start (the cell in hand)
+        Slots64 sq = sn;
+        if (st > 0) prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, Y[st - 1][0], Y[st - 1][1]);
         double z[4] = {0.0, 0.0, 0.0, 0.0};
-        rhs64_vjp<BK, CK>(B, BP, C, CP, F, Gr, acc, K, Y[0], Y[1], Y[2], Y[3], kb[st][0],
-                          kb[st][1], kb[st][2], kb[st][3], z[0], z[1], z[2], z[3]);
+        rhs64_vjp<BK, CK>(B, BP, C, CP, F, sq, Gr, acc, K, Y[st][0], Y[st][1], Yk[0], Yk[1],
+                          kb[st][0], kb[st][1], kb[st][2], kb[st][3], z[0], z[1], z[2], z[3]);
+        if (st == 0) sn = sq;
         if (st > 0) {
             const double hs = st < 3 ? half : dt;
 #pragma unroll
@@ -198,26 +224,79 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
 // The reverse sweep.  Row m is the ray's last finite row: len-1, or len-2
 // when the last row is the all-NaN one (a ray that left the domain, the
 // row `end` holds).  Cotangents: g_end on row len-1 when it is finite, g_x ..
-// g_ky on rows 0..m; none reaches a NaN row.  Lanes past n_rays return at
-// once (no barriers follow).
+// g_ky on rows 0..m; none reaches a NaN row.
+//
+// A workgroup is two waves over the same 64 rays, pipelined over the
+// checkpointed segments, latest first: while wave 1 runs the step adjoints
+// of segment c from one half of the work buffer, wave 0 recomputes segment
+// c-1 (states and stage derivatives, from its checkpoint) into the other
+// half; a barrier separates the rounds.  A segment then costs the longer of
+// the two passes instead of their sum, on SIMDs a one-wave-per-CU launch
+// leaves idle.  Lanes past n_rays stay in the loop, masked, for the
+// barriers.
+constexpr int ADJ_BLOCK = 128;
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = __builtin_elementwise_max(v, __shfl_xor(v, off));
+    return v;
+}
+
 template <int BK, int CK, int MATH>
-__global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
+__global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
 {
     const int64_t nr = a.n_rays;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nr) return;
+    const int wave = (int)(threadIdx.x >> 6);          // 0 recomputes, 1 sweeps back
+    const int64_t rr = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const bool active = rr < nr;
+    const int64_t r = active ? rr : nr - 1;
     const Grid64View B = load_view<Grid64View>(a.B);
     const Grid64View C = load_view<Grid64View>(a.C);
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     const double dt = a.dt, half = a.half, sixth = a.sixth;
     const int stride = a.stride;
     const int64_t ld = a.work_ld;
-    double *wk = a.work + r;
+    const int64_t half_work = (int64_t)stride * 16 * ld;
     const bool dead = all_nan4(a.end[4 * r], a.end[4 * r + 1], a.end[4 * r + 2],
                                a.end[4 * r + 3]);
     const int64_t m = (int64_t)a.len[r] - (dead ? 2 : 1);
+    // segments 0 .. cmax_r of this ray; rounds over the workgroup's largest
+    const int cmax_r = (active && m >= 1) ? (int)((m - 1) / stride) : -1;
+    const int cmax = wave_max(cmax_r);
+    Slots64 sn;                                        // cells for the next evaluation
+    if (wave == 0) {
+        for (int t = 0; t <= cmax + 1; ++t) {
+            const int c = cmax - t;
+            if (c >= 0 && c <= cmax_r) {
+                const int64_t i0 = (int64_t)c * stride;
+                const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
+                double *wk = a.work + (t & 1) * half_work + r;
+                const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
+                double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
+                prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
+                double px = 0.0, py = 0.0;
+                for (int j = 0; j < nseg; ++j) {
+                    double k[3][4], n[4];
+                    rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
+                                             sixth, s, k, n);
+                    double *w = wk + (int64_t)j * 16 * ld;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        w[q * ld] = s[q];
+                        w[(4 + q) * ld] = k[0][q];
+                        w[(8 + q) * ld] = k[1][q];
+                        w[(12 + q) * ld] = k[2][q];
+                        s[q] = n[q];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
     double lam[4] = {0.0, 0.0, 0.0, 0.0};
-    if (m >= 0) {
+    if (active && m >= 0) {
         if (a.g_end && !dead) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
@@ -229,45 +308,37 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
         if (a.g_ky) lam[3] += a.g_ky[o];
     }
     Accs64 acc;
-    for (int64_t c = m >= 1 ? (m - 1) / stride : -1; c >= 0; --c) {
-        const int64_t i0 = c * stride;
-        const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
-        // recompute the segment's states and stage derivatives
-        const double *cp = a.ckpt + (c * 4) * nr + r;
-        double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
-        for (int j = 0; j < nseg; ++j) {
-            double k[3][4], n[4];
-            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, dt, half, sixth, s, k, n);
-            double *w = wk + (int64_t)j * 16 * ld;
+    for (int t = 0; t <= cmax + 1; ++t) {
+        const int c = cmax - t + 1;
+        if (t >= 1 && c <= cmax_r) {
+            const int64_t i0 = (int64_t)c * stride;
+            const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
+            const double *wk = a.work + ((t - 1) & 1) * half_work + r;
+            for (int j = nseg - 1; j >= 0; --j) {
+                const double *w = wk + (int64_t)j * 16 * ld;
+                double k[3][4], y[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                w[q * ld] = s[q];
-                w[(4 + q) * ld] = k[0][q];
-                w[(8 + q) * ld] = k[1][q];
-                w[(12 + q) * ld] = k[2][q];
-                s[q] = n[q];
+                for (int q = 0; q < 4; ++q) {
+                    y[q] = w[q * ld];
+                    k[0][q] = w[(4 + q) * ld];
+                    k[1][q] = w[(8 + q) * ld];
+                    k[2][q] = w[(12 + q) * ld];
+                }
+                if (j == nseg - 1)     // the sweep starts at the segment's last stage point
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, y[0] + k[2][0] * dt,
+                                       y[1] + k[2][1] * dt);
+                rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sn, a.grads, acc, K, dt, half,
+                                         sixth, y, k, lam);
+                const int64_t o = (i0 + j) * nr + r;
+                if (a.g_x) lam[0] += a.g_x[o];
+                if (a.g_y) lam[1] += a.g_y[o];
+                if (a.g_kx) lam[2] += a.g_kx[o];
+                if (a.g_ky) lam[3] += a.g_ky[o];
             }
         }
-        // step adjoints in reverse
-        for (int j = nseg - 1; j >= 0; --j) {
-            const double *w = wk + (int64_t)j * 16 * ld;
-            double k[3][4], y[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                y[q] = w[q * ld];
-                k[0][q] = w[(4 + q) * ld];
-                k[1][q] = w[(8 + q) * ld];
-                k[2][q] = w[(12 + q) * ld];
-            }
-            rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, a.grads, acc, K, dt, half, sixth, y,
-                                     k, lam);
-            const int64_t o = (i0 + j) * nr + r;
-            if (a.g_x) lam[0] += a.g_x[o];
-            if (a.g_y) lam[1] += a.g_y[o];
-            if (a.g_kx) lam[2] += a.g_kx[o];
-            if (a.g_ky) lam[3] += a.g_ky[o];
-        }
+        __syncthreads();
     }
+    if (!active) return;
     if (a.g_x0) a.g_x0[r] = lam[0];
     if (a.g_y0) a.g_y0[r] = lam[1];
     if (a.g_kx0) a.g_kx0[r] = lam[2];
@@ -289,7 +360,8 @@ __global__ void __launch_bounds__(256) rhs64_kernel(Rhs64Args a)
     const Grid64View C = load_view<Grid64View>(a.C);
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     double dx, dy, dkx, dky;
-    rhs64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2],
+    Slots64 sl;                        // empty: the query loads the cell
+    rhs64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, sl, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2],
                         a.s[4 * i + 3], dx, dy, dkx, dky);
     a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
 }
@@ -344,15 +416,14 @@ hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int bl
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int block,
-                          hipStream_t stream)
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, hipStream_t stream)
 {
     if (a.n_rays <= 0) return hipSuccess;
     if (a.stride <= 0 || a.work_ld < a.n_rays) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((a.n_rays + block - 1) / block));
+    const dim3 grid((unsigned)((a.n_rays + 63) / 64));    // 64 rays per workgroup
     dispatch_kinds(bk, ck, math, [&](auto k) {
         using KK = decltype(k);
-        hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(block), 0,
+        hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(ADJ_BLOCK), 0,
                            stream, a);
     });
     return hipGetLastError();

@@ -465,21 +465,35 @@ int build64(const mr_field &f, DevTables &t)
         v.sx = std::fabs(ax[1] - ax[0]); v.sy = std::fabs(ay[1] - ay[0]);
         v.inv_sx = 1.0 / v.sx; v.inv_sy = 1.0 / v.sy;
         v.nxm1 = (double)(f.nx - 1); v.nym1 = (double)(f.ny - 1);
-        v.xspace = ax[1] - ax[0]; v.yspace = ay[1] - ay[0];
-        v.rxs = 1.0 / v.xspace; v.rys = 1.0 / v.yspace;
+        v.rxs = 1.0 / (ax[1] - ax[0]); v.rys = 1.0 / (ay[1] - ay[0]);
         v.oob_from = (uint32_t)(bathy ? first_oob_cell(f.nx, f.ny, (int64_t)f.a.size())
                                       : std::min(first_oob_cell(f.nx, f.ny, (int64_t)f.a.size()),
                                                  first_oob_cell(f.nx, f.ny, (int64_t)f.b.size())));
+        v.last_a = (uint32_t)std::max<int64_t>(0, (int64_t)f.a.size() - 1);
+        v.last_b = (uint32_t)std::max<int64_t>(0, (int64_t)f.b.size() - 1);
+        v.gx0 = (float)ax[0]; v.gy0 = (float)ay[0];
+        v.ginv_x = (float)v.inv_sx; v.ginv_y = (float)v.inv_sy;
+        v.gmax_x = (float)(f.nx - 2); v.gmax_y = (float)(f.ny - 2);
+        // the cell thresholds of the f64 index (p - a0)/|a1 - a0| on these
+        // axes, and the node pairs of each cell
+        std::vector<double2> xt = axis_cells64(ax), yt = axis_cells64(ay);
+        std::vector<double2> xp(ax.size() - 1), yp(ay.size() - 1);
+        for (size_t i = 0; i + 1 < ax.size(); i++) xp[i] = make_double2(ax[i], ax[i + 1]);
+        for (size_t i = 0; i + 1 < ay.size(); i++) yp[i] = make_double2(ay[i], ay[i + 1]);
         double *dax, *day;
-        double2 *dm;
+        double2 *dm, *dxt, *dyt, *dxp, *dyp;
         int rc = dev_upload(t, ax.data(), ax.size(), &dax);
         if (!rc) rc = dev_upload(t, ay.data(), ay.size(), &day);
+        if (!rc) rc = dev_upload(t, xt.data(), xt.size(), &dxt);
+        if (!rc) rc = dev_upload(t, yt.data(), yt.size(), &dyt);
+        if (!rc) rc = dev_upload(t, xp.data(), xp.size(), &dxp);
+        if (!rc) rc = dev_upload(t, yp.data(), yp.size(), &dyp);
         if (!rc) rc = dev_alloc(t, (size_t)((f.nx - 1) * (f.ny - 1)), &dm);
         if (rc) return rc;
         M64BuildArgs ma;
         ma.nx = f.nx; ma.ny = f.ny; ma.ax = dax; ma.ay = day; ma.m = dm;
         MR_HIP_TRY(launch_build_m64(ma, 0));
-        t.p64.ax = dax; t.p64.ay = day; t.p64.m = dm;
+        t.p64.xt = dxt; t.p64.yt = dyt; t.p64.xp = dxp; t.p64.yp = dyp; t.p64.m = dm;
     }
     Grid64View *dv;
     int rc = dev_upload(t, &v, 1, &dv);
@@ -1013,7 +1027,9 @@ int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes)
 {
     if (!bytes) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
     if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
-    *bytes = (int64_t)stride * 16 * n_rays * (int64_t)sizeof(double);
+    // two halves (the recompute and the reverse sweep alternate between
+    // them), each stride steps x 16 doubles x n_rays
+    *bytes = 2 * (int64_t)stride * 16 * n_rays * (int64_t)sizeof(double);
     return MR_OK;
 }
 
@@ -1106,8 +1122,7 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
     a.grads.depth = g_depth;
     a.grads.u = g_u;
     a.grads.v = g_v;
-    const int block = o.block_size ? o.block_size : tuned_block(n_rays);
-    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
+    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math,
                               stream));
     return MR_OK;
 }

@@ -14,12 +14,18 @@
 //   cell    floor, or n-2 on the last node (cartesian_current.rs:283-339)
 //   value   ((a00 + a10 u) + a01 w) + (a11 u) w, u = m01 (y - y_j),
 //           w = m10 (x - x_i)               (interpolator.rs:59-83, in f64)
-//   slopes  (D - A) / (x1 - x0), (B - A) / (y1 - y0)
+//   slopes  (D - A) RN(1/(x1 - x0)), (B - A) RN(1/(y1 - y0))
+//
+//   (the slopes are the cell difference times the rounded reciprocal of the
+//   signed spacing, within an ulp of the reference's quotient)
 //
 // Field values come from kernel arguments (pointers to the raw f64 arrays),
 // so a caller can differentiate with respect to them and change them between
-// calls without rebuilding anything; the per-cell change of basis (m01, m10)
-// depends only on the axes and is tabulated once per field.
+// calls without rebuilding anything; what depends only on the axes (cell
+// thresholds, node pairs, the change of basis m01, m10) is tabulated once per
+// field.  As in the reference mode, each lane holds the cell it is in (a
+// Slot64) and the cell its next evaluation will most likely need, loaded one
+// evaluation ahead.
 //
 // Linearisation.  With u, w the bilinear fractions of the query, the value is
 //   v = wA A + wB B + wC C + wD D,  wA = (1-u)(1-w), wB = u(1-w), wC = u w,
@@ -50,18 +56,21 @@ struct Grid64View {
     int nx, ny, ncx;                      // grid; ncx = nx - 1
     int an;                               // ArrayDepth (an x an)
     uint32_t oob_from;                    // first cell (row-major) with a corner past the data
+    uint32_t last_a, last_b;              // last index of each value array
     double a0x, a0y;                      // first coordinate of each axis
     double sx, sy, inv_sx, inv_sy;        // |a1 - a0| and RN(1/|a1 - a0|)
     double nxm1, nym1;                    // n - 1
-    double xspace, yspace;                // signed a1 - a0 (the slope denominators)
-    double rxs, rys;                      // 1/xspace, 1/yspace (slope weights)
+    double rxs, rys;                      // RN(1/(a1 - a0)), signed: the slope factors
     double h, h0, x0, y0, dhdx, dhdy;     // ConstantDepth / ConstantSlope, widened
     double u, v;                          // ConstantCurrent
+    float gx0, gy0, ginv_x, ginv_y;       // cell guess for prefetch, in f32
+    float gmax_x, gmax_y;                 // (f32)(n - 2)
 };
 
 // Device arrays of a field in fp64 mode (kernel arguments).
 struct Grid64Ptrs {
-    const double *ax, *ay;                // axes in f64 (a bathymetry's f32 axes widened)
+    const double2 *xt, *yt;               // per axis cell i: (T[i], T[i+1]) cell thresholds
+    const double2 *xp, *yp;               // per axis cell i: (a[i], a[i+1]) in f64
     const double2 *m;                     // per cell, row-major: (m01, m10); NaN m01 marks det == 0
     const float *arr;                     // ArrayDepth values
 };
@@ -80,6 +89,20 @@ struct Cell64 {
     double u, w;
 };
 
+// One cell of a gridded field held per lane: its thresholds (the exact cell
+// test, as in the reference mode), node coordinates, change of basis, and
+// the four corners of NV value arrays (depth; or u and v).  valid is false
+// for a cell with a corner past the data and for a slot never loaded;
+// thresholds start as NaN, so such a slot contains no point.
+template <int NV>
+struct Slot64 {
+    double2 tx = {MR_NAN_INIT, MR_NAN_INIT}, ty = {MR_NAN_INIT, MR_NAN_INIT};
+    double2 xp = {0.0, 0.0}, yp = {0.0, 0.0}, m = {0.0, 0.0};
+    double v[NV][4] = {};                 // A, B, C, D
+    uint32_t iA = 0;
+    bool valid = false;
+};
+
 // Cell of (x, y), or ok = false where the reference's index or corner
 // lookup errs (a NaN target included); i1 = j1 = 0 then.
 __device__ __forceinline__ bool locate64(const Grid64View &G, double x, double y, int &i1,
@@ -96,47 +119,110 @@ __device__ __forceinline__ bool locate64(const Grid64View &G, double x, double y
     return ok;
 }
 
-// interpolator::bilinear in f64 on cell (i1, j1) for the value array val,
-// with its corner exit: a target on a node returns that corner (first of
-// A, B, C, D).  (The det == 0 exit is the caller's: m01 is NaN there.)  The
-// corners are read only for a valid cell (ok); the result is NaN otherwise.
-__device__ __forceinline__ Val64 interp64(const Grid64View &G, const double *val, bool ok,
-                                          double x, double y, double xa, double xd, double ya,
-                                          double yb, double m01, double m10, const Cell64 &c)
+template <int NV>
+__device__ __forceinline__ bool slot64_inside(const Slot64<NV> &s, double x, double y)
 {
-    double vA = MR_NAN, vB = MR_NAN, vC = MR_NAN, vD = MR_NAN;
-    if (ok) {                          // the corners exist only in a valid cell
-        vA = val[c.iA]; vB = val[c.iA + (uint32_t)G.nx];
-        vC = val[c.iA + (uint32_t)G.nx + 1u]; vD = val[c.iA + 1u];
+    return s.valid & (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
+}
+
+// Start loading cell (i1, j1) into s.  Corner indices are clamped into the
+// value arrays, so a cell with a corner past the data (never valid) loads
+// in-bounds values that are never used.
+template <int NV>
+__device__ __forceinline__ void slot64_load(const Grid64View &G, const Grid64Ptrs &P,
+                                            const double *va, const double *vb, Slot64<NV> &s,
+                                            int i1, int j1)
+{
+    const uint32_t cell = (uint32_t)j1 * (uint32_t)G.ncx + (uint32_t)i1;
+    s.tx = gldd2(P.xt, (uint32_t)i1 * 16u);
+    s.ty = gldd2(P.yt, (uint32_t)j1 * 16u);
+    s.xp = gldd2(P.xp, (uint32_t)i1 * 16u);
+    s.yp = gldd2(P.yp, (uint32_t)j1 * 16u);
+    s.m = gldd2(P.m, cell * 16u);
+    const uint32_t iA = (uint32_t)G.nx * (uint32_t)j1 + (uint32_t)i1;
+    const uint32_t idx[4] = {iA, iA + (uint32_t)G.nx, iA + (uint32_t)G.nx + 1u, iA + 1u};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        s.v[0][c] = gld(va, __builtin_elementwise_min(idx[c], G.last_a));
+        if (NV > 1) s.v[NV - 1][c] = gld(vb, __builtin_elementwise_min(idx[c], G.last_b));
     }
+    s.iA = iA;
+    s.valid = cell < G.oob_from;
+}
+
+// Start loading the cell that (x, y) most likely lies in (guessed in f32,
+// clamped into the grid; used only to choose what to load).
+template <int NV>
+__device__ __forceinline__ void slot64_prefetch(const Grid64View &G, const Grid64Ptrs &P,
+                                                const double *va, const double *vb,
+                                                Slot64<NV> &s, double x, double y)
+{
+    const int i1 = guess_cell((float)x, G.gx0, G.ginv_x, G.gmax_x);
+    const int j1 = guess_cell((float)y, G.gy0, G.ginv_y, G.gmax_y);
+    slot64_load<NV>(G, P, va, vb, s, i1, j1);
+}
+
+// Make s hold the cell of (x, y): lanes whose slot does not contain the
+// point locate it (the reference's f64 index) and load it, then wait.
+// Returns false where the reference's lookup errs.
+template <int NV>
+__device__ __forceinline__ bool slot64_query(const Grid64View &G, const Grid64Ptrs &P,
+                                             const double *va, const double *vb, Slot64<NV> &s,
+                                             double x, double y)
+{
+    bool ok = slot64_inside(s, x, y);
+    if (__any(!ok)) {
+        const bool miss = !ok;
+        int i1, j1;
+        const bool okn = locate64(G, x, y, i1, j1);
+        if (miss & okn) slot64_load<NV>(G, P, va, vb, s, i1, j1);
+        wait_vmem();
+        ok = ok | (miss & okn);
+    }
+    return ok;
+}
+
+// interpolator::bilinear in f64 on the slot's cell for value array q, with
+// its corner exit (a target on a node returns that corner, first of A, B,
+// C, D), and the cell-difference slopes.  The det == 0 exit is the
+// caller's (m01 is NaN there).
+template <int NV>
+__device__ __forceinline__ Val64 slot64_value(const Grid64View &G, const Slot64<NV> &s, int q,
+                                              double x, double y, Cell64 &c)
+{
+    const double vA = s.v[q][0], vB = s.v[q][1], vC = s.v[q][2], vD = s.v[q][3];
     const double a10 = vB - vA, a01 = vD - vA;
     const double a11 = ((vC - vA) - a10) - a01;
+    c.iA = s.iA;
+    c.u = s.m.x * (y - s.yp.x);
+    c.w = s.m.y * (x - s.xp.x);
     Val64 r;
     r.val = ((vA + a10 * c.u) + a01 * c.w) + (a11 * c.u) * c.w;
-    const bool ex = x == xa, exd = x == xd, eya = y == ya, eyb = y == yb;
+    const bool ex = x == s.xp.x, exd = x == s.xp.y, eya = y == s.yp.x, eyb = y == s.yp.y;
     if ((ex | exd) & (eya | eyb))
         r.val = ex ? (eya ? vA : vB) : (eyb ? vC : vD);
-    r.gx = (vD - vA) / G.xspace;
-    r.gy = (vB - vA) / G.yspace;
-    r.dx = (a01 + a11 * c.u) * m10;
-    r.dy = (a10 + a11 * c.w) * m01;
+    r.gx = (vD - vA) * G.rxs;
+    r.gy = (vB - vA) * G.rys;
+    r.dx = (a01 + a11 * c.u) * s.m.y;
+    r.dy = (a10 + a11 * c.w) * s.m.x;
     return r;
 }
 
-// Corner hit test of interp64 (target exactly on a node of the cell).
-__device__ __forceinline__ bool on_node64(double x, double y, double xa, double xd, double ya,
-                                          double yb)
+// the det == 0 exit: m01 is NaN, unless the target is on a node
+template <int NV>
+__device__ __forceinline__ bool slot64_det_ok(const Slot64<NV> &s, double x, double y)
 {
-    return ((x == xa) | (x == xd)) & ((y == ya) | (y == yb));
+    const bool node = ((x == s.xp.x) | (x == s.xp.y)) & ((y == s.yp.x) | (y == s.yp.y));
+    return node | !__builtin_isnan(s.m.x);
 }
 
 // Bathymetry in fp64 mode.  Returns ok = false where the reference errs;
 // h is NaN (not an error) for a NaN point (cartesian_netcdf3.rs:102-105) and
-// outside an ArrayDepth.  cell.iA is meaningful for BK_GRID only.
+// outside an ArrayDepth.  cell is meaningful for BK_GRID only.
 template <int BK>
 __device__ __forceinline__ bool bathy64(const Grid64View &B, const Grid64Ptrs &P,
-                                        const double *depth, double x, double y, Val64 &h,
-                                        Cell64 &cell)
+                                        const double *depth, Slot64<1> &bs, double x, double y,
+                                        Val64 &h, Cell64 &cell)
 {
     cell.iA = 0; cell.u = 0.0; cell.w = 0.0;
     h.dx = 0.0; h.dy = 0.0;
@@ -160,16 +246,10 @@ __device__ __forceinline__ bool bathy64(const Grid64View &B, const Grid64Ptrs &P
         }
         h.val = (double)P.arr[xi * (uint64_t)B.an + yi];
         return true;
-    } else {
-        int i1, j1;
-        bool ok = locate64(B, x, y, i1, j1);
-        const double xa = P.ax[i1], xd = P.ax[i1 + 1], ya = P.ay[j1], yb = P.ay[j1 + 1];
-        const double2 m = P.m[(uint32_t)j1 * (uint32_t)B.ncx + (uint32_t)i1];
-        cell.iA = (uint32_t)B.nx * (uint32_t)j1 + (uint32_t)i1;
-        cell.u = m.x * (y - ya);
-        cell.w = m.y * (x - xa);
-        h = interp64(B, depth, ok, x, y, xa, xd, ya, yb, m.x, m.y, cell);
-        ok = ok & (on_node64(x, y, xa, xd, ya, yb) | !__builtin_isnan(m.x));
+    } else {                                               // cartesian_netcdf3.rs:98-135
+        bool ok = slot64_query<1>(B, P, depth, depth, bs, x, y);
+        h = slot64_value<1>(B, bs, 0, x, y, cell);
+        ok = ok & slot64_det_ok<1>(bs, x, y);
         if (nn) {
             h.val = MR_NAN; h.gx = MR_NAN; h.gy = MR_NAN;
             ok = true;
@@ -182,8 +262,8 @@ __device__ __forceinline__ bool bathy64(const Grid64View &B, const Grid64Ptrs &P
 // a NaN point errs, cartesian_current.rs:492-495).
 template <int CK>
 __device__ __forceinline__ bool current64(const Grid64View &C, const Grid64Ptrs &P,
-                                          const double *cu, const double *cv, double x, double y,
-                                          Val64 &u, Val64 &v, Cell64 &cell)
+                                          const double *cu, const double *cv, Slot64<2> &cs,
+                                          double x, double y, Val64 &u, Val64 &v, Cell64 &cell)
 {
     cell.iA = 0; cell.u = 0.0; cell.w = 0.0;
     if (CK == CK_CONSTANT) {                               // constant_current.rs:66-76
@@ -191,18 +271,11 @@ __device__ __forceinline__ bool current64(const Grid64View &C, const Grid64Ptrs 
         u.gx = u.gy = u.dx = u.dy = 0.0;
         v.gx = v.gy = v.dx = v.dy = 0.0;
         return true;
-    } else {
-        int i1, j1;
-        bool ok = locate64(C, x, y, i1, j1);
-        const double xa = P.ax[i1], xd = P.ax[i1 + 1], ya = P.ay[j1], yb = P.ay[j1 + 1];
-        const double2 m = P.m[(uint32_t)j1 * (uint32_t)C.ncx + (uint32_t)i1];
-        cell.iA = (uint32_t)C.nx * (uint32_t)j1 + (uint32_t)i1;
-        cell.u = m.x * (y - ya);
-        cell.w = m.y * (x - xa);
-        u = interp64(C, cu, ok, x, y, xa, xd, ya, yb, m.x, m.y, cell);
-        v = interp64(C, cv, ok, x, y, xa, xd, ya, yb, m.x, m.y, cell);
-        ok = ok & (on_node64(x, y, xa, xd, ya, yb) | !__builtin_isnan(m.x));
-        return ok;
+    } else {                                               // cartesian_current.rs:487-542
+        bool ok = slot64_query<2>(C, P, cu, cv, cs, x, y);
+        u = slot64_value<2>(C, cs, 0, x, y, cell);
+        v = slot64_value<2>(C, cs, 1, x, y, cell);
+        return ok & slot64_det_ok<2>(cs, x, y);
     }
 }
 
@@ -211,18 +284,37 @@ struct Fields64 {
     const double *depth, *u, *v;
 };
 
-// WaveRayPath::system in fp64 mode.
+// The cells a lane holds, for the evaluation in hand (q) and the next (n).
+struct Slots64 {
+    Slot64<1> b;
+    Slot64<2> c;
+};
+
+// Start loading, into n, the cells of (x, y) that q does not hold.
+template <int BK, int CK>
+__device__ __forceinline__ void prefetch64(const Grid64View &B, const Grid64Ptrs &BP,
+                                           const Grid64View &C, const Grid64Ptrs &CP,
+                                           const Fields64 &F, const Slots64 &q, Slots64 &n,
+                                           double x, double y)
+{
+    if (BK == BK_GRID && !slot64_inside<1>(q.b, x, y))
+        slot64_prefetch<1>(B, BP, F.depth, F.depth, n.b, x, y);
+    if (CK == CK_GRID && !slot64_inside<2>(q.c, x, y))
+        slot64_prefetch<2>(C, CP, F.u, F.v, n.c, x, y);
+}
+
+// WaveRayPath::system in fp64 mode, with the cells held in sl.
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rhs64(const Grid64View &B, const Grid64Ptrs &BP,
                                       const Grid64View &C, const Grid64Ptrs &CP,
-                                      const Fields64 &F, const FastConsts &K, double x, double y,
-                                      double kx, double ky, double &dx, double &dy, double &dkx,
-                                      double &dky)
+                                      const Fields64 &F, Slots64 &sl, const FastConsts &K,
+                                      double x, double y, double kx, double ky, double &dx,
+                                      double &dy, double &dkx, double &dky)
 {
     Val64 h, u, v;
     Cell64 bc, cc;
-    const bool okb = bathy64<BK>(B, BP, F.depth, x, y, h, bc);
-    const bool okc = current64<CK>(C, CP, F.u, F.v, x, y, u, v, cc);
+    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy, v.gx, v.gy,
                kx, ky, dx, dy, dkx, dky);
 }
@@ -285,15 +377,16 @@ struct Accs64 {
 template <int BK, int CK>
 __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs &BP,
                                           const Grid64View &C, const Grid64Ptrs &CP,
-                                          const Fields64 &F, const Grads64 &Gr, Accs64 &acc,
-                                          const FastConsts &K, double x, double y, double kx,
-                                          double ky, double g0, double g1, double g2, double g3,
-                                          double &bx, double &by, double &bkx, double &bky)
+                                          const Fields64 &F, Slots64 &sl, const Grads64 &Gr,
+                                          Accs64 &acc, const FastConsts &K, double x, double y,
+                                          double kx, double ky, double g0, double g1, double g2,
+                                          double g3, double &bx, double &by, double &bkx,
+                                          double &bky)
 {
     Val64 h, u, v;
     Cell64 bc, cc;
-    const bool okb = bathy64<BK>(B, BP, F.depth, x, y, h, bc);
-    const bool okc = current64<CK>(C, CP, F.u, F.v, x, y, u, v, cc);
+    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     const double hh = h.val;
     const double k2 = __builtin_fma(kx, kx, ky * ky);
     const double rk = rsq_nr(k2);                       // 1/k

@@ -87,7 +87,7 @@ struct AdjointArgs {
     const double *end;                 // last row of each ray (4 per ray)
     const double *g_end;               // cotangent of the last row (4 per ray), may be null
     const double *g_x, *g_y, *g_kx, *g_ky;   // cotangents of the rows, step-major, may be null
-    double *work;                      // [stride][16][work_ld]
+    double *work;                      // [2][stride][16][work_ld]
     int64_t work_ld;                   // >= blocks * block threads
     double *g_x0, *g_y0, *g_kx0, *g_ky0;     // may be null
     Grads64 grads;                     // gradient grids, cleared by the caller
@@ -116,8 +116,7 @@ hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
 hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
 hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,
                           hipStream_t stream);
-hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int block,
-                          hipStream_t stream);
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_rhs64(const Rhs64Args &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_build_m64(const M64BuildArgs &a, hipStream_t stream);
 hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream);

@@ -25,8 +25,11 @@
  *                         solout argument are parity unpinned (ode_solvers
  *                         0.4.0 is not vendored); both solout variants are
  *                         implemented.
- *   fp64-interp providers not in the reference (a differentiable variant);
- *                         pinned by closed forms on linear fields only.
+ *   fp64-interp providers not in the reference (a differentiable variant):
+ *                         pinned by closed forms on linear fields, by
+ *                         scipy's linear interpolator on random fields, at
+ *                         the nodes, and by the reference mode's cell rule
+ *                         and errors (tests/test_oracle_fp64.py).
  */
 #include "mr_oracle.h"
 
@@ -342,7 +345,8 @@ int or_current_and_gradient(const or_current *c, double x, double y,
 }
 
 /* fp64-interp bathymetry: the same index and cell rules in f64 on the widened
- * f32 coordinates, f64 bilinear over f64 depths, unrounded f64 gradient. */
+ * f32 coordinates, f64 bilinear over f64 depths, f64 slopes (the cell
+ * difference times RN(1/spacing), within an ulp of the reference's quotient). */
 int or_bathy_depth_and_gradient64(const or_bathy *b, double px, double py,
                                   double *h, double *hx, double *hy)
 {
@@ -390,10 +394,11 @@ int or_bathy_depth_and_gradient64(const or_bathy *b, double px, double py,
         pts[q][2] = d;
     }
     if (bilinear64((const double (*)[3])pts, px, py, h) != OR_OK) return OR_ERR;
-    double x_space = (double)b->x[1] - (double)b->x[0];
-    double y_space = (double)b->y[1] - (double)b->y[0];
-    *hx = (pts[3][2] - pts[0][2]) / x_space;
-    *hy = (pts[1][2] - pts[0][2]) / y_space;
+    /* slopes: the cell difference times the rounded reciprocal spacing */
+    double rx = 1.0 / ((double)b->x[1] - (double)b->x[0]);
+    double ry = 1.0 / ((double)b->y[1] - (double)b->y[0]);
+    *hx = (pts[3][2] - pts[0][2]) * rx;
+    *hy = (pts[1][2] - pts[0][2]) * ry;
     return OR_OK;
 }
 
@@ -424,12 +429,12 @@ int or_current_and_gradient64(const or_current *c, double x, double y,
     }
     if (bilinear64((const double (*)[3])pu, x, y, u) != OR_OK) return OR_ERR;
     if (bilinear64((const double (*)[3])pv, x, y, v) != OR_OK) return OR_ERR;
-    double x_space = c->x[1] - c->x[0];
-    double y_space = c->y[1] - c->y[0];
-    *ux = (pu[3][2] - pu[0][2]) / x_space;
-    *uy = (pu[1][2] - pu[0][2]) / y_space;
-    *vx = (pv[3][2] - pv[0][2]) / x_space;
-    *vy = (pv[1][2] - pv[0][2]) / y_space;
+    double rx = 1.0 / (c->x[1] - c->x[0]);
+    double ry = 1.0 / (c->y[1] - c->y[0]);
+    *ux = (pu[3][2] - pu[0][2]) * rx;
+    *uy = (pu[1][2] - pu[0][2]) * ry;
+    *vx = (pv[3][2] - pv[0][2]) * rx;
+    *vy = (pv[1][2] - pv[0][2]) * ry;
     return OR_OK;
 }
 

@@ -8,8 +8,9 @@ fp64 mode queries:
     an independent implementation of the same interpolant, and a node
     returns its value exactly;
   - the cell rule and the errors are those of the reference's f64 index
-    (the current's statuses in both modes agree), and the values differ from
-    the reference's f32 interpolation by f32 rounding only.
+    (the current's statuses in both modes agree), the values differ from
+    the reference's f32 interpolation by f32 rounding only, and the slopes
+    (cell difference times RN(1/spacing)) are within an ulp of its quotients.
 Long-double traces (or_fd_end_ld):
   - unperturbed, they end where the f64 oracle's traces end, to the f64
     trace's rounding (below 1e-8 m after 2000 steps);
@@ -103,7 +104,10 @@ def test_fp64_cell_rule_and_errors_match_reference_mode(oracle_mod):
         assert st64 == st32, (px, py)
         if st64 == 0:
             assert abs(v64[0] - v32[0]) <= 1e-6 * scale       # f32 rounding of the point and arithmetic
-            assert v64[2:] == v32[2:]                 # slopes are f64 in both modes
+            # slopes: the same f64 cell difference, times RN(1/spacing) here and
+            # divided by the spacing in the reference mode (within an ulp)
+            for a64, a32 in zip(v64[2:], v32[2:]):
+                assert abs(a64 - a32) <= 2.3e-16 * abs(a32)
 
 
 def _deep_case(O, h=2000.0):
