@@ -235,6 +235,27 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current,
                     double *g_x0, double *g_y0, double *g_kx0, double *g_ky0,
                     double *g_depth, double *g_u, double *g_v);
 
+/* ---- result writers (host only) -------------------------------------- */
+
+/* RayResult::from(SolverResult) then as_json (src/ray_result.rs:124-151,
+ * :74-76): one ray's rows up to the first row with a NaN component, as
+ * {"t_vec":[..],"x_vec":[..],"y_vec":[..],"kx_vec":[..],"ky_vec":[..]} with
+ * serde_json's f64 text.  Row r is t[r] and x[r*stride] .. ky[r*stride]
+ * (stride = n_rays for one ray of mr_trace_many's step-major output, offset
+ * to that ray).  *len receives the text's length; the text and a NUL are
+ * written to buf only when cap > *len (call with buf NULL to size it). */
+int mr_ray_result_json(const double *t, const double *x, const double *y, const double *kx,
+                       const double *ky, int64_t n_rows, int64_t stride, char *buf,
+                       int64_t cap, int64_t *len);
+
+/* output_or_append_to_tsv_file (src/ray.rs:218-243): appends to path
+ * (created if missing) the header "t x y kx ky", one line per row up to the
+ * first with a NaN x, each value followed by a space and the line by " \n",
+ * then "END"; values as Rust's Display prints f64.  Rows as above. */
+int mr_ray_result_append_tsv(const char *path, const double *t, const double *x,
+                             const double *y, const double *kx, const double *ky,
+                             int64_t n_rows, int64_t stride);
+
 #ifdef __cplusplus
 }
 #endif

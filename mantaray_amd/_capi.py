@@ -37,7 +37,7 @@ EXPORTS = (
     "mr_current_constant", "mr_field_free", "mr_field_kind", "mr_field_grid_shape",
     "mr_field_copy_grid", "mr_field_upload", "mr_num_steps", "mr_trace_many",
     "mr_single_ray", "mr_rhs_many", "mr_checkpoint_count", "mr_adjoint_workspace_size",
-    "mr_trace_fp64", "mr_adjoint_many",
+    "mr_trace_fp64", "mr_adjoint_many", "mr_ray_result_json", "mr_ray_result_append_tsv",
 )
 
 
@@ -109,6 +109,9 @@ _SIGS = {
                                        ctypes.c_int32, ctypes.POINTER(Options), _vp, _vp, _vp,
                                        _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp]),
+    "mr_ray_result_json": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64,
+                                          _i64p]),
+    "mr_ray_result_append_tsv": (ctypes.c_int, [_cs, _vp, _vp, _vp, _vp, _vp, _i64, _i64]),
 }
 
 _lib = None

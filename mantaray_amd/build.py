@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmantaray_hip.so")
-SOURCES = ["mr_kernels.hip", "mr_adjoint.hip", "mr_capi.cpp", "mr_netcdf3.cpp"]
+SOURCES = ["mr_kernels.hip", "mr_adjoint.hip", "mr_capi.cpp", "mr_netcdf3.cpp", "mr_results.cpp"]
 ARCH = os.environ.get("MANTARAY_OFFLOAD_ARCH", "gfx950")
 
 

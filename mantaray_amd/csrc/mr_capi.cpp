@@ -111,6 +111,11 @@ int64_t first_oob_cell(int64_t nx, int64_t ny, int64_t len)
 
 }  // namespace
 
+namespace mr {
+// the error reporting of this file, for the other host translation units
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace mr
+
 struct mr_field {
     int32_t kind = 0;
     float h = 0, h0 = 0, x0 = 0, y0 = 0, dhdx = 0, dhdy = 0;
