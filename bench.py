@@ -16,6 +16,11 @@ endpoints): the fp64-interpolation forward alone, and that forward with its
 checkpoints followed by the reverse sweep (gradients with respect to the
 initial states and the u, v grids), each timed with events on its stream.
 
+The inversion case (2048 x 2048 grid at 100 m, depth 30 m, 262,144 rays,
+3000 steps) runs 20 descent iterations of forward + adjoint from a zero
+current against endpoints traced through the true current, and reports the
+time per iteration, memory and the recovered current's RMS error.
+
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
 its own event-timed duration; HBM bytes per launch from the rocprofv3
@@ -53,6 +58,7 @@ def parse():
     p.add_argument("--block", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    p.add_argument("--no-inversion", action="store_true", help="skip the inversion run")
     p.add_argument("--cpu-rays", type=int, default=16384,
                    help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
@@ -177,6 +183,51 @@ def adjoint_cost(dev, reps=5):
                     "only; fwd+bwd = that forward with checkpoints, then the reverse sweep"}
 
 
+def inversion_cost(dev, iterations=20):
+    """The inversion workload: 20 descent iterations of forward + adjoint on
+    the device, from a zero current, against endpoints traced through the
+    true current (see mantaray_amd.inversion)."""
+    import numpy as np
+    import torch
+
+    from mantaray_amd import adjoint, inversion, workloads
+    cfg = workloads.eddy_inversion()
+    b, c = workloads.make_fields(cfg)
+    b.upload(0)
+    c.upload(0)
+    T, dt, stride = cfg["t_end"], cfg["dt"], cfg["stride"]
+    ic = [torch.tensor(cfg[k], dtype=torch.float64, device=dev) for k in ("x0", "y0", "kx0", "ky0")]
+    ut = torch.tensor(cfg["current"]["u"], device=dev)
+    vt = torch.tensor(cfg["current"]["v"], device=dev)
+    obs = adjoint.trace_fp64(b, c, *ic, 0.0, T, dt, u=ut.view(-1), v=vt.view(-1))["end"][:, :2]
+    obs = obs.clone()
+    z = torch.zeros_like(ut)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    free0, total = torch.cuda.mem_get_info(dev)
+    sigma, first = 100.0, 0.5
+    r = inversion.invert(b, c, ic, obs, T, dt, u0=z, v0=z, iterations=iterations,
+                         stride=stride, sigma_cells=sigma, first_step=first, method="bb",
+                         u_true=ut, v_true=vt)
+    h = r["history"]
+    per_iter = [x["seconds"] for x in h[:-1]]
+    nr = obs.shape[0]
+    return {"workload": "eddy_inversion", "grid": "2048x2048 @ 100 m", "rays": nr,
+            "rk4_steps": int(T / dt), "checkpoint_stride": stride, "iterations": iterations,
+            "method": f"gradient descent, Barzilai-Borwein steps, gradient smoothed by a "
+                      f"Gaussian of {sigma:g} cells",
+            "seconds_per_iteration": float(np.median(per_iter)),
+            "seconds_total": float(sum(per_iter)),
+            "ray_steps_per_iteration": nr * int(T / dt),
+            "loss_initial": h[0]["loss"], "loss_final": h[-1]["loss"],
+            "rms_error_initial": h[0]["rms_error"], "rms_error_final": h[-1]["rms_error"],
+            "rms_true_current": float(torch.sqrt((ut ** 2 + vt ** 2).mean())),
+            "peak_torch_bytes": int(torch.cuda.max_memory_allocated(dev)),
+            "checkpoint_bytes": int(adjoint.checkpoint_count(0.0, T, dt, stride) * 4 * nr * 8),
+            "workspace_bytes": int(adjoint.workspace_bytes(nr, stride)),
+            "device_bytes_in_use_at_start": int(total - free0)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -259,8 +310,15 @@ def main():
                      "hbm_GBps": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9,
                      "hbm_peak_GBps": 8000.0},
     }
+    # the trajectories of the forward measurement (10.5 GB) are not needed by
+    # the adjoint and inversion measurements, whose memory use is reported
+    del out, ic
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_adjoint:
         res["adjoint"] = adjoint_cost(dev)
+    if not args.no_inversion:
+        res["inversion"] = inversion_cost(dev)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_rays, nr))
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]

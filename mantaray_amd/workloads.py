@@ -23,6 +23,13 @@ machine with no dependence on numpy's RNG.
                    ~ U[0, 2 pi); dt = 1 s, 2000 steps; endpoint targets
                    launch point + U[-20, 20] km per axis.  The adjoint
                    workload: loss sum_r |(x, y)_r(T) - target_r|^2.
+  eddy_inversion() 2048 x 2048 grid at 100 m, constant depth 30 m, true
+                   current the sum of 20 Gaussian eddies (peak speed
+                   ~ U[0.5, 1.5] m/s, radius ~ U[10, 50] km); 262,144 rays at
+                   uniform points at least 50 km inside the edges, period
+                   ~ U[8, 14] s, direction ~ U[0, 2 pi); dt = 1 s, 3000
+                   steps.  The inversion workload: recover the current from
+                   the rays' endpoints, starting from zero.
 The wavenumber magnitude solves the linear dispersion relation
 sigma^2 = g k tanh(k h) (g = 9.8, sigma = 2 pi / T) at the launch depth by
 Newton's method.
@@ -172,6 +179,28 @@ def eddy_adjoint(n_rays: int = 16384, seed: int = 3, n_steps: int = 2000) -> dic
         "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
         "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
         "target_x": tx, "target_y": ty,
+        "t0": 0.0, "t_end": float(n_steps), "dt": 1.0, "stride": 50,
+    }
+
+
+def eddy_inversion(n_rays: int = 262144, seed: int = 4, n_steps: int = 3000) -> dict:
+    n = 2048
+    dx = 100.0
+    xd = np.arange(n) * dx
+    rs = Stream(seed)
+    u, v = eddies(xd, xd, rs, peak=(0.5, 1.5))
+    margin = 50000.0
+    x0 = rs.uniform(n_rays, margin, xd[-1] - margin)
+    y0 = rs.uniform(n_rays, margin, xd[-1] - margin)
+    period = rs.uniform(n_rays, 8.0, 14.0)
+    theta = rs.uniform(n_rays, 0.0, 2.0 * np.pi)
+    h = 30.0
+    k = wavenumber(2.0 * np.pi / period, np.full(n_rays, h))
+    return {
+        "name": "eddy_inversion",
+        "bathy": {"constant": h},
+        "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
+        "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
         "t0": 0.0, "t_end": float(n_steps), "dt": 1.0, "stride": 50,
     }
 
