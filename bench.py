@@ -21,6 +21,11 @@ The inversion case (2048 x 2048 grid at 100 m, depth 30 m, 262,144 rays,
 current against endpoints traced through the true current, and reports the
 time per iteration, memory and the recovered current's RMS error.
 
+The batched case (64 scenes of 256^2 to 2048^2 cells, 2^20 rays, up to
+10,000 steps each) times one persistent launch of all scenes (launch shape
+from the autotuner, and the same launch replayed from a HIP graph) against
+one launch per scene.
+
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
 its own event-timed duration; HBM bytes per launch from the rocprofv3
@@ -59,6 +64,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--no-inversion", action="store_true", help="skip the inversion run")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched-scenes run")
     p.add_argument("--cpu-rays", type=int, default=16384,
                    help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
@@ -228,6 +234,71 @@ def inversion_cost(dev, iterations=20):
             "device_bytes_in_use_at_start": int(total - free0)}
 
 
+def batch_cost(dev):
+    """The batched workload: 64 scenes (grid sides 256-2048 at 250 m, random
+    depth and eddies, 2^20 rays of uneven lifetime, steps 0.5-2 s, up to
+    10,000 steps) in one persistent launch, against one launch per scene;
+    launch shape from the autotuner; the launch replayed from a HIP graph."""
+    import numpy as np
+    import torch
+
+    from mantaray_amd import core, workloads
+    from mantaray_amd.autotune import Autotuner
+    from mantaray_amd.batch import Batch
+    t0 = time.perf_counter()
+    cfgs = workloads.batched_scenes()
+    gen_s = time.perf_counter() - t0
+    scenes = []
+    for cfg in cfgs:
+        b, c = workloads.make_fields(cfg)
+        scenes.append({"bathymetry": b, "current": c,
+                       **{k: cfg[k] for k in ("x0", "y0", "kx0", "ky0", "t0", "t_end", "dt")}})
+    t0 = time.perf_counter()
+    batch = Batch(scenes)
+    build_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream(dev)
+    best, table = batch.autotune("batched_scenes_seed2", tuner=Autotuner(
+        os.path.join(ROOT, "gpurun_out", "autotune.json")), force=True)
+    batch.run()
+    torch.cuda.synchronize()
+    ray_steps = int(sum(int((o["len"].to(torch.int64) - 1).sum()) for o in batch.results()))
+    t_batch = _event_ms(batch.run, stream, 3)
+    ics = [[torch.tensor(sc[k], dtype=torch.float64, device=dev) for k in
+            ("x0", "y0", "kx0", "ky0")] for sc in scenes]
+
+    def per_scene():
+        for sc, ic in zip(scenes, ics):
+            core.trace_many_device(sc["bathymetry"], sc["current"], *ic, sc["t0"], sc["t_end"],
+                                   sc["dt"], trajectories=False, stream=stream)
+
+    per_scene()
+    t_scenes = _event_ms(per_scene, stream, 3)
+    s = torch.cuda.Stream(dev)
+    s.wait_stream(stream)
+    with torch.cuda.stream(s):
+        batch.run(stream=s)
+    stream.wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        batch.run(stream=s)
+    t_graph = _event_ms(g.replay, stream, 3)
+    n_rays, bound = batch.info()
+    batch.close()
+    return {"workload": "batched_scenes", "scenes": len(scenes),
+            "grid_sides": sorted({int(sc["bathy"]["x"].size) for sc in cfgs}), "rays": n_rays,
+            "ray_steps": ray_steps, "ray_steps_bound": bound,
+            "persistent_ms": t_batch, "persistent_ray_steps_per_s": ray_steps / (t_batch * 1e-3),
+            "graph_replay_ms": t_graph,
+            "per_scene_launches_ms": t_scenes,
+            "per_scene_ray_steps_per_s": ray_steps / (t_scenes * 1e-3),
+            "speedup_vs_per_scene": t_scenes / t_batch,
+            "autotuned": best,
+            "autotune_table_ms": [[c["block_size"], c["blocks_per_cu"], t] for c, t in table],
+            "scene_generation_s": gen_s, "tables_build_s": build_s,
+            "outputs": "row count and last row of every ray"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -319,6 +390,8 @@ def main():
         res["adjoint"] = adjoint_cost(dev)
     if not args.no_inversion:
         res["inversion"] = inversion_cost(dev)
+    if not args.no_batch:
+        res["batch"] = batch_cost(dev)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(cfg, min(args.cpu_rays, nr))
         res["speedup_vs_cpu"] = value / res["cpu_baseline"]["value"]

@@ -83,7 +83,7 @@ typedef struct mr_options {
     int32_t memory;        /* MR_MEM_* */
     int32_t device;        /* HIP device ordinal; -1 = current device */
     int32_t block_size;    /* threads per block, multiple of 64; 0 = tuned */
-    int32_t reserved;
+    int32_t blocks_per_cu; /* mr_batch_run: resident workgroups per CU; 0 = tuned */
     void   *stream;        /* hipStream_t for MR_MEM_DEVICE; NULL = null stream */
 } mr_options;
 
@@ -234,6 +234,41 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current,
                     void *work, int64_t work_bytes,
                     double *g_x0, double *g_y0, double *g_kx0, double *g_ky0,
                     double *g_depth, double *g_u, double *g_v);
+
+/* ---- batched scenes: one persistent launch ----------------------------
+ *
+ * No counterpart in the reference, which traces one scene per call
+ * (ManyRays::trace_many, src/ray.rs:98-127).  Many scenes, each with its own
+ * fields, step, step count and rays, are traced by one launch of a fixed
+ * number of workgroups that take rays from a shared queue: a lane whose ray
+ * has ended takes the next ray of any scene.  Each ray's result is what
+ * mr_trace_many gives for its scene (row count and last row; bitwise the
+ * same numbers).  All scenes must share the bathymetry kind and the current
+ * kind.  Device memory only. */
+typedef struct mr_scene {
+    const mr_field *bathy, *current;
+    int64_t n_rays;
+    const double *x0, *y0, *kx0, *ky0;   /* [n_rays] each, device */
+    double t0, t_end, dt;
+    int32_t *len_out;                    /* [n_rays], device */
+    double *end_out;                     /* [4*n_rays], device: last row (x, y, kx, ky) */
+} mr_scene;
+
+typedef struct mr_batch mr_batch;
+
+/* Builds the scenes' device tables and the batch's queue and descriptors
+ * (the only allocations); scenes[] is copied. */
+int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *opt,
+                    mr_batch **out);
+/* Enqueues the trace on opt->stream: a queue reset and one persistent
+ * launch, nothing allocated and no host synchronisation, so the call can be
+ * captured in a HIP graph and replayed.  Calls on one batch must not
+ * overlap (they share the queue).  opt: math, solout, block_size and
+ * blocks_per_cu (0 = defaults) */
+int mr_batch_run(mr_batch *batch, const mr_options *opt);
+/* Total ray count and an upper bound of the ray-steps (sum of n_rays * n). */
+int mr_batch_info(const mr_batch *batch, int64_t *n_rays, int64_t *max_ray_steps);
+int mr_batch_free(mr_batch *batch);
 
 /* ---- result writers (host only) -------------------------------------- */
 

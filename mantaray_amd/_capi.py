@@ -38,7 +38,19 @@ EXPORTS = (
     "mr_field_copy_grid", "mr_field_upload", "mr_num_steps", "mr_trace_many",
     "mr_single_ray", "mr_rhs_many", "mr_checkpoint_count", "mr_adjoint_workspace_size",
     "mr_trace_fp64", "mr_adjoint_many", "mr_ray_result_json", "mr_ray_result_append_tsv",
+    "mr_batch_create", "mr_batch_run", "mr_batch_info", "mr_batch_free",
 )
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("bathy", ctypes.c_void_p), ("current", ctypes.c_void_p),
+        ("n_rays", ctypes.c_int64),
+        ("x0", ctypes.c_void_p), ("y0", ctypes.c_void_p),
+        ("kx0", ctypes.c_void_p), ("ky0", ctypes.c_void_p),
+        ("t0", ctypes.c_double), ("t_end", ctypes.c_double), ("dt", ctypes.c_double),
+        ("len_out", ctypes.c_void_p), ("end_out", ctypes.c_void_p),
+    ]
 
 
 class MantarayError(RuntimeError):
@@ -58,7 +70,7 @@ class Options(ctypes.Structure):
         ("memory", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
     ]
 
@@ -112,6 +124,11 @@ _SIGS = {
     "mr_ray_result_json": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64,
                                           _i64p]),
     "mr_ray_result_append_tsv": (ctypes.c_int, [_cs, _vp, _vp, _vp, _vp, _vp, _i64, _i64]),
+    "mr_batch_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(Scene),
+                                       ctypes.POINTER(Options), ctypes.POINTER(_vp)]),
+    "mr_batch_run": (ctypes.c_int, [_vp, ctypes.POINTER(Options)]),
+    "mr_batch_info": (ctypes.c_int, [_vp, _i64p, _i64p]),
+    "mr_batch_free": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

@@ -130,6 +130,16 @@ struct mr_field {
     std::map<int, std::unique_ptr<DevTables>> dev;
 };
 
+struct mr_batch {
+    int device = 0;
+    int bk = 0, ck = 0;
+    int64_t n_rays = 0, max_ray_steps = 0;
+    std::unique_ptr<DevTables> mem;     // descriptors, ray -> scene map, queue head
+    SceneDesc *scenes = nullptr;
+    int32_t *ray_scene = nullptr;
+    unsigned long long *next = nullptr;
+};
+
 namespace {
 
 int validate_axes_f32(const std::vector<float> &x, const char *name)
@@ -702,6 +712,7 @@ void mr_options_init(mr_options *opt)
     opt->memory = MR_MEM_HOST;
     opt->device = -1;
     opt->block_size = 0;
+    opt->blocks_per_cu = 0;
     opt->stream = nullptr;
 }
 
@@ -1014,6 +1025,111 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     MR_HIP_TRY(launch_rhs(a, bathy_kind_of(bathy), current_kind_of(current), o.math, 0));
     MR_HIP_TRY(hipDeviceSynchronize());
     if (n) MR_HIP_TRY(hipMemcpy(ds, d_ds, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return MR_OK;
+}
+
+int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *opt,
+                    mr_batch **out)
+{
+    if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (n_scenes < 0 || (n_scenes > 0 && !scenes))
+        return fail(MR_ERR_INVALID_ARGUMENT, "invalid scene array");
+    mr_options o;
+    int rc = resolve_options(opt, o);
+    if (rc) return rc;
+    if (o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "batches take device memory (MR_MEM_DEVICE)");
+    if (o.precision != MR_PREC_REFERENCE)
+        return fail(MR_ERR_UNSUPPORTED, "batches run in MR_PREC_REFERENCE");
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    std::unique_ptr<mr_batch> b(new mr_batch);
+    MR_HIP_TRY(hipGetDevice(&b->device));
+    b->mem.reset(new DevTables);
+    std::vector<SceneDesc> desc((size_t)n_scenes);
+    std::vector<int32_t> ray_scene;
+    for (int32_t i = 0; i < n_scenes; i++) {
+        const mr_scene &sc = scenes[i];
+        if ((rc = check_pair(sc.bathy, sc.current))) return rc;
+        const int bk = bathy_kind_of(sc.bathy), ck = current_kind_of(sc.current);
+        if (i == 0) { b->bk = bk; b->ck = ck; }
+        else if (bk != b->bk || ck != b->ck)
+            return fail(MR_ERR_INVALID_ARGUMENT, "scenes of one batch must share field kinds");
+        if (sc.n_rays < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative ray count");
+        if (sc.n_rays > 0 && (!sc.x0 || !sc.y0 || !sc.kx0 || !sc.ky0 || !sc.len_out || !sc.end_out))
+            return fail(MR_ERR_INVALID_ARGUMENT, "null scene arrays");
+        int64_t n;
+        if ((rc = mr_num_steps(sc.t0, sc.t_end, sc.dt, &n))) return rc;
+        const DevTables *bt, *ct;
+        if ((rc = field_tables(sc.bathy, &bt)) || (rc = field_tables(sc.current, &ct))) return rc;
+        SceneDesc &d = desc[(size_t)i];
+        d.B = bt->view; d.C = ct->view; d.bp = bt->bp; d.cp = ct->cp;
+        d.dt = sc.dt; d.half = sc.dt / 2.0; d.sixth = sc.dt / 6.0;   // Rk4::new, Rk4::step
+        d.n_steps = n;
+        d.ray0 = b->n_rays;
+        d.x0 = sc.x0; d.y0 = sc.y0; d.kx0 = sc.kx0; d.ky0 = sc.ky0;
+        d.len_out = sc.len_out; d.end_out = sc.end_out;
+        b->n_rays += sc.n_rays;
+        b->max_ray_steps += sc.n_rays * n;
+        ray_scene.insert(ray_scene.end(), (size_t)sc.n_rays, i);
+    }
+    if (b->n_rays >= (int64_t(1) << 31)) return fail(MR_ERR_UNSUPPORTED, "too many rays");
+    if ((rc = dev_upload(*b->mem, desc.data(), desc.size(), &b->scenes)) ||
+        (rc = dev_upload(*b->mem, ray_scene.data(), ray_scene.size(), &b->ray_scene)) ||
+        (rc = dev_alloc(*b->mem, 1, &b->next)))
+        return rc;
+    MR_HIP_TRY(hipDeviceSynchronize());
+    *out = b.release();
+    return MR_OK;
+}
+
+int mr_batch_run(mr_batch *b, const mr_options *opt)
+{
+    if (!b) return fail(MR_ERR_INVALID_ARGUMENT, "null batch");
+    mr_options o;
+    int rc = resolve_options(opt, o);
+    if (rc) return rc;
+    if (o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "batches take device memory (MR_MEM_DEVICE)");
+    if (o.blocks_per_cu < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative blocks_per_cu");
+    DeviceGuard g(b->device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    if (b->n_rays == 0) return MR_OK;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) !=
+            hipSuccess || cus <= 0)
+        cus = 256;
+    const int block = o.block_size ? o.block_size : 256;
+    const int per_cu = o.blocks_per_cu ? o.blocks_per_cu : 1;
+    const int64_t needed = (b->n_rays + block - 1) / block;
+    const int blocks = (int)std::min<int64_t>(needed, (int64_t)cus * per_cu);
+    hipStream_t stream = (hipStream_t)o.stream;
+    BatchArgs a;
+    a.scenes = b->scenes;
+    a.ray_scene = b->ray_scene;
+    a.n_rays = b->n_rays;
+    a.next = b->next;
+    a.solout = o.solout;
+    MR_HIP_TRY(launch_clear_counter(b->next, stream));
+    MR_HIP_TRY(launch_batch(a, b->bk, b->ck, o.math, block, blocks, stream));
+    return MR_OK;
+}
+
+int mr_batch_info(const mr_batch *b, int64_t *n_rays, int64_t *max_ray_steps)
+{
+    if (!b) return fail(MR_ERR_INVALID_ARGUMENT, "null batch");
+    if (n_rays) *n_rays = b->n_rays;
+    if (max_ray_steps) *max_ray_steps = b->max_ray_steps;
+    return MR_OK;
+}
+
+int mr_batch_free(mr_batch *b)
+{
+    if (b) {
+        DeviceGuard g(b->device);
+        delete b;
+    }
     return MR_OK;
 }
 

@@ -24,6 +24,28 @@ struct TraceArgs {
     int solout;                        // 0 state, 1 next derivative
 };
 
+// One scene of a batched trace, in device memory.
+struct SceneDesc {
+    const void *B, *C;                 // BathyView / CurrentView in device memory
+    BathyPtrs bp;
+    CurrentPtrs cp;
+    double dt, half, sixth;
+    int64_t n_steps;
+    int64_t ray0;                      // first global ray of the scene
+    const double *x0, *y0, *kx0, *ky0; // the scene's initial conditions
+    int32_t *len_out;                  // the scene's outputs
+    double *end_out;
+};
+
+// A persistent batched trace: rays of every scene taken from one queue.
+struct BatchArgs {
+    const SceneDesc *scenes;
+    const int32_t *ray_scene;          // scene of each global ray
+    int64_t n_rays;                    // over all scenes
+    unsigned long long *next;          // queue head, cleared before the launch
+    int solout;
+};
+
 struct RhsArgs {
     const void *B;
     const void *C;
@@ -112,6 +134,9 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
                         hipStream_t stream);
 hipError_t launch_rhs(const RhsArgs &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_fill_time(double t0, double dt, int64_t n, double *t, hipStream_t stream);
+hipError_t launch_batch(const BatchArgs &a, int bk, int ck, int math, int block, int blocks,
+                        hipStream_t stream);
+hipError_t launch_clear_counter(unsigned long long *p, hipStream_t stream);
 hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
 hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
 hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,

@@ -28,7 +28,60 @@ namespace mr {
 // record is in registers one evaluation after it was asked for and a ray
 // crossing into a new cell does not wait for memory; a wrong guess only
 // costs a blocking reload in the query.
-//
+
+// One RK4 step of one lane from (x, y, kx, ky): the new state in o, and
+// stop_before when (MR_SOLOUT_NEXT_DERIV) the first stage derivative is
+// all-NaN, i.e. the ray already ended on the row it starts from.  bn, cn:
+// the records prefetched for the next evaluation; (px, py): the latest
+// dx/dt, dy/dt, for the guess.
+template <int BK, int CK, int MATH>
+__device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP, BathySlot &bn,
+                                         const CurrentView &C, const CurrentPtrs &CP,
+                                         CurrentSlot &cn, const FastConsts &K, double &px,
+                                         double &py, double dt, double half, double sixth,
+                                         int solout, double x, double y, double kx, double ky,
+                                         double o[4], bool &stop_before)
+{
+    constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
+    double sx = x, sy = y, skx = kx, sky = ky;   // stage state
+    double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
+    stop_before = false;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const double hs = st < 2 ? half : dt;
+        BathySlot bq = bn;
+        CurrentSlot cq = cn;
+        // the next evaluation is at x + k hs (stages 1-3) or at the
+        // next step's start, about x + k dt; only lanes whose guess
+        // leaves the cell in hand load a record
+        const double gx = x + px * hs, gy = y + py * hs;
+        if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
+            bathy_prefetch(B, BP, bn, gx, gy);
+        if (PF_C && !current_inside(cq, gx, gy))
+            current_prefetch(C, CP, cn, gx, gy);
+        double dx, dy, dkx, dky;
+        rhs<BK, CK, MATH>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        px = dx; py = dy;
+        if (st == 0) {
+            // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
+            // just pushed: an all-NaN k1 here means the ray already
+            // ended on that row.
+            stop_before = solout == 1 && all_nan4(dx, dy, dkx, dky);
+            ax = dx; ay = dy; akx = dkx; aky = dky;
+        } else if (st < 3) {
+            // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
+            ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
+            akx = __builtin_fma(dkx, 2.0, akx); aky = __builtin_fma(dky, 2.0, aky);
+        } else {
+            ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
+        }
+        sx = x + dx * hs; sy = y + dy * hs;
+        skx = kx + dkx * hs; sky = ky + dky * hs;
+    }
+    o[0] = x + ax * sixth; o[1] = y + ay * sixth;
+    o[2] = kx + akx * sixth; o[3] = ky + aky * sixth;
+}
+
 // Lanes past n_rays repeat the last ray (same inputs, same arithmetic, so
 // their stores write the same bytes to the same addresses): every lane
 // stores every row, with no branch around the stores.  OUT: 0 no
@@ -71,46 +124,14 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (!__any(alive)) break;
         double ox = MR_NAN, oy = MR_NAN, okx = MR_NAN, oky = MR_NAN;
         if (alive) {
-            double sx = x, sy = y, skx = kx, sky = ky;   // stage state
-            double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
-            bool stop_before = false;
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                const double hs = st < 2 ? half : dt;
-                BathySlot bq = bn;
-                CurrentSlot cq = cn;
-                // the next evaluation is at x + k hs (stages 1-3) or at the
-                // next step's start, about x + k dt; only lanes whose guess
-                // leaves the cell in hand load a record
-                const double gx = x + px * hs, gy = y + py * hs;
-                if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
-                    bathy_prefetch(B, a.bp, bn, gx, gy);
-                if (PF_C && !current_inside(cq, gx, gy))
-                    current_prefetch(C, a.cp, cn, gx, gy);
-                double dx, dy, dkx, dky;
-                rhs<BK, CK, MATH>(B, a.bp, bq, C, a.cp, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
-                px = dx; py = dy;
-                if (st == 0) {
-                    // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
-                    // just pushed: an all-NaN k1 here means the ray already
-                    // ended on that row.
-                    stop_before = a.solout == 1 && all_nan4(dx, dy, dkx, dky);
-                    ax = dx; ay = dy; akx = dkx; aky = dky;
-                } else if (st < 3) {
-                    // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
-                    ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
-                    akx = __builtin_fma(dkx, 2.0, akx); aky = __builtin_fma(dky, 2.0, aky);
-                } else {
-                    ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
-                }
-                sx = x + dx * hs; sy = y + dy * hs;
-                skx = kx + dkx * hs; sky = ky + dky * hs;
-            }
+            double o[4];
+            bool stop_before;
+            rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
+                                   a.solout, x, y, kx, ky, o, stop_before);
             if (stop_before) {
                 alive = false;
             } else {
-                ox = x + ax * sixth; oy = y + ay * sixth;
-                okx = kx + akx * sixth; oky = ky + aky * sixth;
+                ox = o[0]; oy = o[1]; okx = o[2]; oky = o[3];
                 x = ox; y = oy; kx = okx; ky = oky;
                 len = s + 1;
                 if (all_nan4(ox, oy, okx, oky)) alive = false;
@@ -146,6 +167,100 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         a.end_out[4 * r + 0] = x; a.end_out[4 * r + 1] = y;
         a.end_out[4 * r + 2] = kx; a.end_out[4 * r + 3] = ky;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent trace: many scenes (each its own fields, step and step
+// count) in one launch of a fixed number of workgroups.  Every lane owns one
+// ray at a time; when its ray ends (edge exit, end of its steps), the lane
+// takes the next ray of the queue, of whatever scene, so lanes stay busy
+// while rays of uneven lifetime come and go (ray compaction at lane level).
+// The lanes of a wave that need a ray take consecutive queue entries with one
+// atomic add per wave.  Each lane holds its scene's field views, pointers and
+// step in registers, and its record slots are emptied when it changes rays.
+// Outputs: each ray's row count and last row, as trace_kernel writes them.
+template <int BK, int CK, int MATH>
+__global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
+{
+    constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    const int lane = (int)(threadIdx.x & 63);
+    BathyView B{};
+    CurrentView C{};
+    BathyPtrs bp{};
+    CurrentPtrs cp{};
+    BathySlot bn;
+    CurrentSlot cn;
+    double x = 0.0, y = 0.0, kx = 0.0, ky = 0.0, px = 0.0, py = 0.0;
+    double dt = 0.0, half = 0.0, sixth = 0.0;
+    int64_t local = 0, n_steps = 0, step = 0, len = 1;
+    int32_t *len_out = nullptr;
+    double *end_out = nullptr;
+    bool have = false, done = false;
+    for (;;) {
+        const bool need = !have && !done;
+        const uint64_t mask = __ballot(need);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.next, (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader);
+            if (need) {
+                const int64_t id =
+                    (int64_t)base + __popcll(mask & ((uint64_t(1) << lane) - uint64_t(1)));
+                if (id < a.n_rays) {
+                    const SceneDesc *S = a.scenes + a.ray_scene[id];
+                    B = load_view<BathyView>(S->B);
+                    C = load_view<CurrentView>(S->C);
+                    bp = S->bp;
+                    cp = S->cp;
+                    dt = S->dt; half = S->half; sixth = S->sixth;
+                    n_steps = S->n_steps;
+                    local = id - S->ray0;
+                    len_out = S->len_out;
+                    end_out = S->end_out;
+                    x = S->x0[local]; y = S->y0[local]; kx = S->kx0[local]; ky = S->ky0[local];
+                    step = 0; len = 1; px = 0.0; py = 0.0;
+                    bn = BathySlot{};
+                    cn = CurrentSlot{};
+                    if (PF_B) bathy_prefetch(B, bp, bn, x, y);
+                    if (PF_C) current_prefetch(C, cp, cn, x, y);
+                    have = true;
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (!__any(have)) break;
+        if (have) {
+            bool finished = step >= n_steps;
+            if (!finished) {
+                double o[4];
+                bool stop_before;
+                rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
+                                       a.solout, x, y, kx, ky, o, stop_before);
+                ++step;
+                if (stop_before) {
+                    finished = true;
+                } else {
+                    x = o[0]; y = o[1]; kx = o[2]; ky = o[3];
+                    len = step + 1;
+                    finished = all_nan4(x, y, kx, ky) || step >= n_steps;
+                }
+            }
+            if (finished) {
+                len_out[local] = (int32_t)len;
+                end_out[4 * local + 0] = x; end_out[4 * local + 1] = y;
+                end_out[4 * local + 2] = kx; end_out[4 * local + 3] = ky;
+                have = false;
+            }
+        }
+    }
+}
+
+__global__ void clear_counter_kernel(unsigned long long *p)
+{
+    if (threadIdx.x == 0) *p = 0ull;
 }
 
 template <int BK, int CK, int MATH>
@@ -267,6 +382,25 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
             hipLaunchKernelGGL((trace_kernel<K::bk, K::ck, K::math, 2>), grid, dim3(block), 0,
                                stream, a);
     });
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const BatchArgs &a, int bk, int ck, int math, int block, int blocks,
+                        hipStream_t stream)
+{
+    if (a.n_rays <= 0) return hipSuccess;
+    if (block < 64 || block > 256 || block % 64 || blocks < 1) return hipErrorInvalidValue;
+    dispatch_kinds(bk, ck, math, [&](auto k) {
+        using KK = decltype(k);
+        hipLaunchKernelGGL((batch_kernel<KK::bk, KK::ck, KK::math>), dim3((unsigned)blocks),
+                           dim3(block), 0, stream, a);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_counter(unsigned long long *p, hipStream_t stream)
+{
+    hipLaunchKernelGGL(clear_counter_kernel, dim3(1), dim3(64), 0, stream, p);
     return hipGetLastError();
 }
 

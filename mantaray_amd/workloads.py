@@ -23,6 +23,7 @@ machine with no dependence on numpy's RNG.
                    ~ U[0, 2 pi); dt = 1 s, 2000 steps; endpoint targets
                    launch point + U[-20, 20] km per axis.  The adjoint
                    workload: loss sum_r |(x, y)_r(T) - target_r|^2.
+  batched_scenes() 64 scenes of the batched workload (see the function).
   eddy_inversion() 2048 x 2048 grid at 100 m, constant depth 30 m, true
                    current the sum of 20 Gaussian eddies (peak speed
                    ~ U[0.5, 1.5] m/s, radius ~ U[10, 50] km); 262,144 rays at
@@ -142,19 +143,20 @@ def eddies(x: np.ndarray, y: np.ndarray, rs: Stream, n_eddies: int = 20,
     """u, v [j, i] of a sum of Gaussian eddies: stream function
     psi = A exp(-r^2 / (2 R^2)), (u, v) = (-dpsi/dy, dpsi/dx), with
     A = +-peak R e^(1/2) so each eddy's speed peaks at r = R."""
-    X, Y = np.meshgrid(x, y)
-    u = np.zeros_like(X)
-    v = np.zeros_like(X)
+    u = np.zeros((y.size, x.size))
+    v = np.zeros((y.size, x.size))
     xc = rs.uniform(n_eddies, x[0], x[-1])
     yc = rs.uniform(n_eddies, y[0], y[-1])
     pk = rs.uniform(n_eddies, *peak)
     R = rs.uniform(n_eddies, *radius)
     sense = np.where(rs.uniform(n_eddies) < 0.5, -1.0, 1.0)
     for e in range(n_eddies):
-        A = sense[e] * pk[e] * R[e] * np.exp(0.5)
-        g = np.exp(-((X - xc[e]) ** 2 + (Y - yc[e]) ** 2) / (2.0 * R[e] ** 2)) / R[e] ** 2
-        u += A * (Y - yc[e]) * g
-        v -= A * (X - xc[e]) * g
+        # the Gaussian factors into exp(-dx^2/2R^2) exp(-dy^2/2R^2): outer products
+        A = sense[e] * pk[e] * R[e] * np.exp(0.5) / R[e] ** 2
+        gx = np.exp(-(x - xc[e]) ** 2 / (2.0 * R[e] ** 2))
+        gy = np.exp(-(y - yc[e]) ** 2 / (2.0 * R[e] ** 2))
+        u += np.outer(A * (y - yc[e]) * gy, gx)
+        v -= np.outer(A * gy, (x - xc[e]) * gx)
     return u, v
 
 
@@ -203,6 +205,71 @@ def eddy_inversion(n_rays: int = 262144, seed: int = 4, n_steps: int = 3000) -> 
         "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
         "t0": 0.0, "t_end": float(n_steps), "dt": 1.0, "stride": 50,
     }
+
+
+def normals(rs: Stream, n: int) -> np.ndarray:
+    """n standard normal draws (Box-Muller on pairs of uniforms)."""
+    m = (n + 1) // 2
+    u1 = 1.0 - rs.uniform(m)                           # in (0, 1]
+    u2 = rs.uniform(m)
+    r = np.sqrt(-2.0 * np.log(u1))
+    return np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])[:n]
+
+
+def gaussian_random_field(n: int, dx: float, rs: Stream, mean: float, sd: float,
+                          corr: float) -> np.ndarray:
+    """White noise on an n x n grid filtered by a Gaussian of standard
+    deviation corr (periodic, by FFT), scaled to the given mean and sd."""
+    w = normals(rs, n * n).reshape(n, n)
+    f = np.fft.fftfreq(n, d=dx) * 2.0 * np.pi
+    tf = np.exp(-0.5 * (corr * f) ** 2)
+    g = np.fft.ifft2(np.fft.fft2(w) * tf[:, None] * tf[None, :]).real
+    g = (g - g.mean()) / g.std()
+    return mean + sd * g
+
+
+def batched_scenes(n_scenes: int = 64, total_rays: int = 1 << 20, seed: int = 2,
+                   max_steps: int = 10000) -> list:
+    """Scenes of the batched workload: grid side ~ {256, 512, 1024, 2048} at
+    250 m; depth a Gaussian random field (mean 2000 m, sd 800 m, correlation
+    length 30 km, clipped at 1 m); current the sum of 20 Gaussian eddies
+    (peak 0-2 m/s, radius 10-50 km); rays per scene log-uniform in
+    [1024, 65536], rescaled to total_rays; step ~ U[0.5, 2] s; steps
+    ~ U[1000, max_steps]; rays at uniform interior points with period
+    ~ U[6, 16] s and direction ~ U[0, 2 pi).  Rays leave small scenes early,
+    so lifetimes are uneven."""
+    rs = Stream(seed)
+    sides = np.array([256, 512, 1024, 2048])[
+        np.minimum((rs.uniform(n_scenes) * 4).astype(int), 3)]
+    raw = np.exp(rs.uniform(n_scenes, np.log(1024.0), np.log(65536.0)))
+    counts = np.maximum(1, np.floor(raw / raw.sum() * total_rays)).astype(np.int64)
+    counts[np.argmax(counts)] += total_rays - counts.sum()
+    dts = rs.uniform(n_scenes, 0.5, 2.0)
+    steps = np.floor(rs.uniform(n_scenes, 1000.0, max_steps + 1.0)).astype(np.int64)
+    dx = 250.0
+    scenes = []
+    for i in range(n_scenes):
+        n = int(sides[i])
+        xd = np.arange(n) * dx
+        depth = np.maximum(gaussian_random_field(n, dx, rs, 2000.0, 800.0, 30000.0), 1.0)
+        u, v = eddies(xd, xd, rs)
+        nr = int(counts[i])
+        x0 = rs.uniform(nr, dx, xd[-1] - dx)
+        y0 = rs.uniform(nr, dx, xd[-1] - dx)
+        period = rs.uniform(nr, 6.0, 16.0)
+        theta = rs.uniform(nr, 0.0, 2.0 * np.pi)
+        fx = np.clip(np.rint(x0 / dx).astype(int), 0, n - 1)
+        fy = np.clip(np.rint(y0 / dx).astype(int), 0, n - 1)
+        k = wavenumber(2.0 * np.pi / period, depth[fy, fx])
+        dt = float(dts[i])
+        scenes.append({
+            "name": f"batched_{i}",
+            "bathy": {"x": xd.astype(np.float32), "y": xd.astype(np.float32), "depth": depth},
+            "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
+            "x0": x0, "y0": y0, "kx0": k * np.cos(theta), "ky0": k * np.sin(theta),
+            "t0": 0.0, "t_end": float(steps[i]) * dt, "dt": dt,
+        })
+    return scenes
 
 
 def make_fields(cfg: dict):
