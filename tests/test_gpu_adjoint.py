@@ -404,3 +404,49 @@ def test_adjoint_eddy_workload(mr, oracle_mod, torch):
         err = abs(gu[idx] - fd) / abs(fd)
         print(f"eddy d/du[{idx}]: adjoint {gu[idx]:.12e} long-double fd {fd:.12e} rel {err:.2e}")
         assert err <= 1e-6, (idx, err)
+
+
+def test_adjoint_edge_cases(mr, torch):
+    """Zero steps; a stride longer than the trace; NaN and out-of-grid initial
+    states (a NaN row at once: only row 0's cotangent reaches them); no rays."""
+    from mantaray_amd import adjoint
+    b, c, ic, tx, ty, vals = _small_case(mr, torch, n=64)
+    ic = [t.clone() for t in ic]
+    ic[0][3] = float("nan")                     # NaN initial x
+    ic[1][5] = 1e6                              # outside the grid
+    rng = np.random.default_rng(2)
+    # zero steps: the only row is the initial state
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, 0.0, 1.0, stride=4, trajectories=True, **vals)
+    assert fwd["ckpt"].shape[0] == 1 and int(fwd["len"].max()) == 1
+    ge = _dev(torch, rng.normal(size=(64, 4)))
+    g = adjoint.adjoint(b, c, fwd, 0.0, 0.0, 1.0, 4, g_end=ge, **vals)
+    torch.cuda.synchronize()
+    want = ge.cpu().numpy()
+    end = fwd["end"].cpu().numpy()
+    dead = np.isnan(end).all(axis=1)
+    for q, k in enumerate(("x0", "y0", "kx0", "ky0")):
+        got = g[k].cpu().numpy()
+        assert np.array_equal(got[~dead], want[~dead, q]), k
+    # 40 steps, stride 1000 (one segment), rows cotangent on every row
+    T = 40.0
+    rows = {k: _dev(torch, rng.normal(size=(41, 64))) for k in ("x", "y")}
+    f1 = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=1000, trajectories=True, **vals)
+    f2 = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=3, trajectories=True, **vals)
+    g1 = adjoint.adjoint(b, c, f1, 0.0, T, 1.0, 1000, g_rows=rows, **vals)
+    g2 = adjoint.adjoint(b, c, f2, 0.0, T, 1.0, 3, g_rows=rows, **vals)
+    torch.cuda.synchronize()
+    ln = f1["len"].cpu().numpy()
+    assert ln[3] == 2 and ln[5] == 2            # a NaN row straight away
+    for k in ("x0", "y0", "kx0", "ky0"):
+        assert torch.equal(g1[k], g2[k]), k
+    # the dead-at-once rays get row 0's cotangent only
+    for r_ in (3, 5):
+        assert g1["x0"][r_].item() == rows["x"][0, r_].item()
+        assert g1["y0"][r_].item() == rows["y"][0, r_].item()
+        assert g1["kx0"][r_].item() == 0.0 and g1["ky0"][r_].item() == 0.0
+    # no rays: nothing traced, the grid gradients are cleared
+    empty = [t[:0].clone() for t in ic]
+    f0 = adjoint.trace_fp64(b, c, *empty, 0.0, T, 1.0, stride=5, **vals)
+    g0 = adjoint.adjoint(b, c, f0, 0.0, T, 1.0, 5, **vals)
+    torch.cuda.synchronize()
+    assert g0["x0"].numel() == 0 and float(g0["u"].abs().sum()) == 0.0
