@@ -176,6 +176,10 @@ def adjoint_cost(dev, reps=5):
     t_ref = _event_ms(fwd_ref, stream, reps)
     f = state["f"]
     ray_steps = int((f["len"].to(torch.int64) - 1).sum().item())
+    from mantaray_amd.autotune import Autotuner
+    best, table = adjoint.tune_stride(b, c, *ic, 0.0, T, dt, u=u, v=v, force=True,
+                                      tuner=Autotuner(os.path.join(ROOT, "gpurun_out",
+                                                                   "autotune.json")))
     return {"workload": "eddy_adjoint", "grid": "512x512 @ 250 m", "rays": nr,
             "rk4_steps": int(T / dt), "checkpoint_stride": stride,
             "loss": "sum_r |(x, y)_r(T) - target_r|^2",
@@ -185,6 +189,8 @@ def adjoint_cost(dev, reps=5):
             "ray_steps": ray_steps,
             "checkpoint_bytes": int(f["ckpt"].numel() * 8),
             "workspace_bytes": int(work.numel()),
+            "autotuned_stride": best,
+            "stride_table_ms": table,
             "note": "fwd = the fp64-interpolation forward the adjoint differentiates, endpoints "
                     "only; fwd+bwd = that forward with checkpoints, then the reverse sweep"}
 

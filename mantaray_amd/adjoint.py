@@ -161,6 +161,43 @@ def adjoint(bathymetry: Bathymetry, current: Current, fwd: dict, t0: float, t_en
     return res
 
 
+def tune_stride(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0, t0: float,
+                t_end: float, dt: float, *, depth=None, u=None, v=None,
+                candidates=(10, 25, 50, 100, 200), memory_budget: int | None = None,
+                key: str | None = None, math: str = "fast", force: bool = False, tuner=None):
+    """Pick the checkpoint stride by timing forward-with-checkpoints plus the
+    reverse sweep (endpoint loss, initial-state and grid gradients) for each
+    candidate whose checkpoints and work buffer fit memory_budget bytes
+    (default: no limit).  A short stride stores more checkpoints; a long one
+    a larger work buffer.  Returns (stride, [(candidate, ms), ...])."""
+    import torch
+
+    from .autotune import Autotuner
+    nr = min(t.numel() for t in (x0, y0, kx0, ky0))
+
+    def mem(k):
+        return checkpoint_count(t0, t_end, dt, k) * 4 * nr * 8 + workspace_bytes(nr, k)
+
+    cands = [{"stride": int(k)} for k in candidates
+             if memory_budget is None or mem(k) <= memory_budget]
+    if not cands:
+        raise ValueError("no checkpoint stride fits the memory budget")
+    want = tuple(n for n, ok in (("x0", True), ("depth", bathymetry.kind == C.MR_BATHY_GRID),
+                                 ("u", current.kind == C.MR_CURRENT_GRID),
+                                 ("v", current.kind == C.MR_CURRENT_GRID)) if ok)
+
+    def run(stride):
+        f = trace_fp64(bathymetry, current, x0, y0, kx0, ky0, t0, t_end, dt, depth=depth, u=u,
+                       v=v, stride=stride, math=math)
+        g = torch.ones_like(f["end"])
+        adjoint(bathymetry, current, f, t0, t_end, dt, stride, depth=depth, u=u, v=v, g_end=g,
+                want=want, math=math)
+
+    k = key or f"stride|{nr}|{t0}|{t_end}|{dt}|{math}|{memory_budget}"
+    best, table = (tuner or Autotuner()).tune(k, cands, run, force=force)
+    return best["stride"], [(c["stride"], t) for c, t in table]
+
+
 @dataclass
 class TraceSpec:
     """Non-tensor arguments of RayTrace."""

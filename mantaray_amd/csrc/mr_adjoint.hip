@@ -350,6 +350,209 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
     }
 }
 
+// The reverse sweep with the linearisations on waves of their own.  A
+// step's four stage linearisations (field queries, dispersion terms, the
+// matrix of rhs64_lin) do not depend on the cotangent chain, only their
+// application does.  A workgroup is 2 + NL waves over the same 64 rays, in
+// rounds separated by a barrier:
+//   wave 0          recomputes one step of segment c-1 (as in adjoint_kernel);
+//   waves 1 .. NL   linearise one step of segment c, 4/NL stages each
+//                   (latest first), in reverse step order, into a
+//                   double-buffered LDS ring; each loads the next round's
+//                   step from the work buffer while it works on this one;
+//   wave NL + 1     applies the records of the step linearised the round
+//                   before (the chain of rk4_step_adjoint), adds the row
+//                   cotangents and sums the grid cotangents.
+// Segment c's K steps take K rounds; segment cmax is recomputed alone first
+// and the chain trails by one round: (cmax + 2) K + 1 rounds in all.  A
+// round costs about 4/NL stage linearisations, against eight in the
+// two-wave sweep.  Meant for launches of at most one workgroup per CU, where
+// the sweep is latency-bound (the LDS ring holds one workgroup per CU).  NL
+// = 2 is the one launched: with NL = 4 the six waves share SIMDs and the
+// 256 registers a wave then gets spill.
+template <int BK, int CK, int MATH, int NL>
+__global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs a)
+{
+    constexpr int SPW = 4 / NL;                        // stages per linearising wave
+    __shared__ double ring[2][4][LIN_N][64];
+    const int64_t nr = a.n_rays;
+    const int wave = (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t rr = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = rr < nr;
+    const int64_t r = active ? rr : nr - 1;
+    const Grid64View B = load_view<Grid64View>(a.B);
+    const Grid64View C = load_view<Grid64View>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    const double dt = a.dt, half = a.half, sixth = a.sixth;
+    const int stride = a.stride;
+    const int64_t ld = a.work_ld;
+    const int64_t half_work = (int64_t)stride * 16 * ld;
+    const bool dead = all_nan4(a.end[4 * r], a.end[4 * r + 1], a.end[4 * r + 2],
+                               a.end[4 * r + 3]);
+    const int64_t m = (int64_t)a.len[r] - (dead ? 2 : 1);
+    const int cmax_r = (active && m >= 1) ? (int)((m - 1) / stride) : -1;
+    const int cmax = wave_max(cmax_r);
+    const int rounds = cmax >= 0 ? (cmax + 2) * stride + 1 : 0;
+    const int chain = NL + 1;
+    Slots64 sn;                                        // cells for the next evaluation
+    double s[4] = {0.0, 0.0, 0.0, 0.0};                // wave 0: the state being advanced
+    double px = 0.0, py = 0.0;
+    double nxt[16];                                    // linearising waves: the next step's data
+    bool pre = false;
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};              // chain wave: the cotangent
+    Accs64 acc;
+    if (wave == chain && active && m >= 0) {
+        if (a.g_end && !dead) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
+        }
+        const int64_t o = m * nr + r;
+        if (a.g_x) lam[0] += a.g_x[o];
+        if (a.g_y) lam[1] += a.g_y[o];
+        if (a.g_kx) lam[2] += a.g_kx[o];
+        if (a.g_ky) lam[3] += a.g_ky[o];
+    }
+    for (int rho = 0; rho < rounds; ++rho) {
+        if (wave == 0) {
+            const int c = cmax - rho / stride, j = rho % stride;
+            if (c >= 0 && c <= cmax_r) {
+                const int64_t i0 = (int64_t)c * stride;
+                if (j < m - i0) {
+                    if (j == 0) {
+                        const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
+                        s[0] = cp[0]; s[1] = cp[nr]; s[2] = cp[2 * nr]; s[3] = cp[3 * nr];
+                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
+                        px = 0.0; py = 0.0;
+                    }
+                    double k[3][4], n[4];
+                    rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
+                                             sixth, s, k, n);
+                    double *w = a.work + (c & 1) * half_work + r + (int64_t)j * 16 * ld;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        w[q * ld] = s[q];
+                        w[(4 + q) * ld] = k[0][q];
+                        w[(8 + q) * ld] = k[1][q];
+                        w[(12 + q) * ld] = k[2][q];
+                        s[q] = n[q];
+                    }
+                }
+            }
+        } else if (wave < chain) {
+            const int qq = rho / stride;
+            const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
+            if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
+                const double *wb = a.work + (c & 1) * half_work + r;
+                double d[16];
+                if (pre) {
+#pragma unroll
+                    for (int f = 0; f < 16; ++f) d[f] = nxt[f];
+                } else {
+                    const double *w = wb + (int64_t)j * 16 * ld;
+#pragma unroll
+                    for (int f = 0; f < 16; ++f) d[f] = w[f * ld];
+                }
+                // start loading the next round's step (same segment)
+                pre = j >= 1;
+                if (pre) {
+                    const double *w = wb + (int64_t)(j - 1) * 16 * ld;
+#pragma unroll
+                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
+                }
+                // the four stage points y, y + k1 dt/2, y + k2 dt/2, y + k3 dt
+                double P[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    P[0][q] = d[q];
+                    P[1][q] = d[q] + d[4 + q] * half;
+                    P[2][q] = d[q] + d[8 + q] * half;
+                    P[3][q] = d[q] + d[12 + q] * dt;
+                }
+                // this wave's stages, latest first (the wave selects values,
+                // the indices stay compile-time)
+                const int first = 3 - (wave - 1) * SPW;
+#pragma unroll
+                for (int h = 0; h < SPW; ++h) {
+                    const int st = first - h;
+                    double Y[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        Y[q] = st == 3 ? P[3][q] : st == 2 ? P[2][q] : st == 1 ? P[1][q] : P[0][q];
+                    Slots64 sq = sn;
+                    // the next point: this wave's next stage, or, after its
+                    // last, its first stage one step earlier (guessed one
+                    // step back along k1)
+                    double gx, gy;
+                    if (h + 1 < SPW) {
+                        const int sx_ = st - 1;
+                        gx = sx_ == 2 ? P[2][0] : sx_ == 1 ? P[1][0] : P[0][0];
+                        gy = sx_ == 2 ? P[2][1] : sx_ == 1 ? P[1][1] : P[0][1];
+                    } else {
+                        const double fx = first == 3 ? P[3][0] : first == 2 ? P[2][0]
+                                        : first == 1 ? P[1][0] : P[0][0];
+                        const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
+                                        : first == 1 ? P[1][1] : P[0][1];
+                        gx = fx - d[4] * dt;
+                        gy = fy - d[5] * dt;
+                    }
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
+                    double R[LIN_N];
+                    rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
+                    if (h + 1 == SPW) sn = sq;
+#pragma unroll
+                    for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
+                }
+            } else {
+                pre = false;
+            }
+        } else if (rho >= 1) {
+            const int q1 = (rho - 1) / stride;
+            const int c = cmax - q1 + 1, j = stride - 1 - (rho - 1) % stride;
+            if (q1 >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
+                double kb[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double w = lam[q] * sixth;
+                    kb[0][q] = w; kb[3][q] = w;
+                    kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
+                }
+#pragma unroll
+                for (int st = 3; st >= 0; --st) {
+                    double R[LIN_N];
+#pragma unroll
+                    for (int f = 0; f < LIN_N; ++f) R[f] = ring[(rho - 1) & 1][st][f][lane];
+                    double z[4];
+                    lin_apply<BK, CK>(R, B, C, a.grads, acc, kb[st], z);
+                    if (st > 0) {
+                        const double hs = st < 3 ? half : dt;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) kb[st - 1][q] += z[q] * hs;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) lam[q] += z[q];
+                }
+                const int64_t o = ((int64_t)c * stride + j) * nr + r;
+                if (a.g_x) lam[0] += a.g_x[o];
+                if (a.g_y) lam[1] += a.g_y[o];
+                if (a.g_kx) lam[2] += a.g_kx[o];
+                if (a.g_ky) lam[3] += a.g_ky[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (wave != chain || !active) return;
+    if (a.g_x0) a.g_x0[r] = lam[0];
+    if (a.g_y0) a.g_y0[r] = lam[1];
+    if (a.g_kx0) a.g_kx0[r] = lam[2];
+    if (a.g_ky0) a.g_ky0[r] = lam[3];
+    if (BK == BK_GRID && a.grads.depth) acc_flush(acc.h, a.grads.depth, (uint32_t)B.nx);
+    if (CK == CK_GRID) {
+        if (a.grads.u) acc_flush(acc.u, a.grads.u, (uint32_t)C.nx);
+        if (a.grads.v) acc_flush(acc.v, a.grads.v, (uint32_t)C.nx);
+    }
+}
+
 // WaveRayPath::system in fp64 mode at a batch of states
 template <int BK, int CK, int MATH>
 __global__ void __launch_bounds__(256) rhs64_kernel(Rhs64Args a)
@@ -416,15 +619,21 @@ hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int bl
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, hipStream_t stream)
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int waves,
+                          hipStream_t stream)
 {
     if (a.n_rays <= 0) return hipSuccess;
     if (a.stride <= 0 || a.work_ld < a.n_rays) return hipErrorInvalidValue;
+    if (waves != 2 && waves != 4) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((a.n_rays + 63) / 64));    // 64 rays per workgroup
     dispatch_kinds(bk, ck, math, [&](auto k) {
         using KK = decltype(k);
-        hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(ADJ_BLOCK), 0,
-                           stream, a);
+        if (waves == 4)
+            hipLaunchKernelGGL((adjoint_lin_kernel<KK::bk, KK::ck, KK::math, 2>), grid,
+                               dim3(4 * 64), 0, stream, a);
+        else
+            hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid,
+                               dim3(ADJ_BLOCK), 0, stream, a);
     });
     return hipGetLastError();
 }

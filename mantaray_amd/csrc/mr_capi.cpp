@@ -540,7 +540,7 @@ int check_pair(const mr_field *b, const mr_field *c)
     return MR_OK;
 }
 
-int resolve_options(const mr_options *opt, mr_options &o)
+int resolve_options(const mr_options *opt, mr_options &o, int max_block = 256)
 {
     mr_options_init(&o);
     if (opt) {
@@ -556,8 +556,9 @@ int resolve_options(const mr_options *opt, mr_options &o)
         return fail(MR_ERR_INVALID_ARGUMENT, "unknown solout variant");
     if (o.memory != MR_MEM_HOST && o.memory != MR_MEM_DEVICE)
         return fail(MR_ERR_INVALID_ARGUMENT, "unknown memory kind");
-    if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64))
-        return fail(MR_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+    if (o.block_size != 0 && (o.block_size < 64 || o.block_size > max_block || o.block_size % 64))
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "block_size must be a multiple of 64 up to " + std::to_string(max_block));
     return MR_OK;
 }
 
@@ -1243,7 +1244,20 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
     a.grads.depth = g_depth;
     a.grads.u = g_u;
     a.grads.v = g_v;
-    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math,
+    // four waves per 64 rays while the launch has at most one workgroup per
+    // CU (latency-bound), two beyond; block_size 128 or 256 forces two or four
+    int cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    int waves = (n_rays + 63) / 64 <= cus ? 4 : 2;
+    if (o.block_size == 128 || o.block_size == 256)
+        waves = o.block_size / 64;
+    else if (o.block_size != 0)
+        return fail(MR_ERR_INVALID_ARGUMENT, "mr_adjoint_many: block_size must be 0, 128 or 256");
+    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, waves,
                               stream));
     return MR_OK;
 }

@@ -368,6 +368,114 @@ struct Accs64 {
     CornerAcc h, u, v;
 };
 
+// The linearisation of rhs64 at one stage point, as a record of LIN_N
+// doubles, so that the cotangent chain can run apart from the (costlier)
+// field queries and dispersion terms:
+//   [0, 16)   M, row-major: the state cotangent is z = M g for a cotangent g
+//             of (dx/dt, dy/dt, dkx/dt, dky/dt)
+//   [16, 20)  hb: the depth-value cotangent is hb . g
+//   20, 21, 22  c, kx, ky: the slope cotangents (g2 c, g3 c), (-g2 kx, -g3 kx),
+//             (-g2 ky, -g3 ky) of depth, u and v
+//   23, 24    bathymetry cell fractions (u, w); 25, 26 the current's
+//   27        the two cells' corner-A indices, bit-packed (all ones: none)
+constexpr int LIN_N = 28;
+
+__device__ __forceinline__ double pack_cells(uint32_t b, uint32_t c)
+{
+    return __builtin_bit_cast(double, ((uint64_t)c << 32) | (uint64_t)b);
+}
+__device__ __forceinline__ void unpack_cells(double v, uint32_t &b, uint32_t &c)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    b = (uint32_t)u;
+    c = (uint32_t)(u >> 32);
+}
+
+template <int BK, int CK>
+__device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs &BP,
+                                          const Grid64View &C, const Grid64Ptrs &CP,
+                                          const Fields64 &F, Slots64 &sl, const FastConsts &K,
+                                          double x, double y, double kx, double ky,
+                                          double R[LIN_N])
+{
+    Val64 h, u, v;
+    Cell64 bc, cc;
+    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    const double hh = h.val;
+    const double k2 = __builtin_fma(kx, kx, ky * ky);
+    const double rk = rsq_nr(k2);
+    const double k = k2 * rk;
+    const double ex = kx * rk, ey = ky * rk;
+    const double q = k * hh;
+    double e, em;
+    exp_pair(2.0 * q, e, em, K);
+    const double ope = 1.0 + e;
+    const double D = rcp_nr(em * ope);
+    const double iope = em * D;
+    const double T = em * iope;
+    const double S = (4.0 * e) * (iope * iope);
+    const double invT = (ope * ope) * D;
+    const double gkT = (K.g * k) * T;
+    const double rs = rsq_nr(gkT);
+    const double sigma = gkT * rs;
+    const double N = __builtin_fma(q, S, T);
+    const double cg = K.half_g * N * rs;
+    const double c = (-2.0 * k) * e * sigma * D;
+    const double sig_h = K.half_g * (k * k) * S * rs;
+    const double Nq = 2.0 * S * (1.0 - q * T);
+    const double cg_k = K.half_g * rs * (hh * Nq - N * cg * rs);
+    const double cg_h = K.half_g * rs * (k * Nq - N * sig_h * rs);
+    const double coth2 = 0.5 * (T + invT);
+    const double c_k = c * (rk + cg * rs - 2.0 * hh * coth2);
+    const double c_h = c * (sig_h * rs - 2.0 * k * coth2);
+    const double tk = cg * rk;
+    const double hb[4] = {cg_h * ex, cg_h * ey, c_h * h.gx, c_h * h.gy};
+    const double a = cg_k - tk;
+    // rows of M (the partial derivatives of f, transposed)
+    R[0] = hb[0] * h.dx + u.dx; R[1] = hb[1] * h.dx + v.dx;
+    R[2] = hb[2] * h.dx;        R[3] = hb[3] * h.dx;
+    R[4] = hb[0] * h.dy + u.dy; R[5] = hb[1] * h.dy + v.dy;
+    R[6] = hb[2] * h.dy;        R[7] = hb[3] * h.dy;
+    R[8] = a * ex * ex + tk;    R[9] = a * ex * ey;
+    R[10] = c_k * ex * h.gx - u.gx; R[11] = c_k * ex * h.gy - u.gy;
+    R[12] = a * ey * ex;        R[13] = a * ey * ey + tk;
+    R[14] = c_k * ey * h.gx - v.gx; R[15] = c_k * ey * h.gy - v.gy;
+    R[16] = hb[0]; R[17] = hb[1]; R[18] = hb[2]; R[19] = hb[3];
+    R[20] = c; R[21] = kx; R[22] = ky;
+    R[23] = bc.u; R[24] = bc.w; R[25] = cc.u; R[26] = cc.w;
+    // a state of a finite row never errs; the marks keep a misuse in bounds
+    R[27] = pack_cells((BK == BK_GRID && okb) ? bc.iA : 0xFFFFFFFFu,
+                       (CK == CK_GRID && okc) ? cc.iA : 0xFFFFFFFFu);
+}
+
+// Applies a linearisation record R to the cotangent g: z = M g, and the
+// field-value cotangents to the accumulators.
+template <int BK, int CK>
+__device__ __forceinline__ void lin_apply(const double R[LIN_N], const Grid64View &B,
+                                          const Grid64View &C, const Grads64 &Gr, Accs64 &acc,
+                                          const double g[4], double z[4])
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        z[i] = ((R[4 * i] * g[0] + R[4 * i + 1] * g[1]) + R[4 * i + 2] * g[2]) + R[4 * i + 3] * g[3];
+    uint32_t ib, ic;
+    unpack_cells(R[27], ib, ic);
+    if (BK == BK_GRID && Gr.depth && ib != 0xFFFFFFFFu) {
+        const double hbar = ((R[16] * g[0] + R[17] * g[1]) + R[18] * g[2]) + R[19] * g[3];
+        Cell64 cell{ib, R[23], R[24]};
+        acc_add(acc.h, Gr.depth, (uint32_t)B.nx, cell, B.rxs, B.rys, hbar, g[2] * R[20],
+                g[3] * R[20]);
+    }
+    if (CK == CK_GRID && ic != 0xFFFFFFFFu) {
+        Cell64 cell{ic, R[25], R[26]};
+        if (Gr.u) acc_add(acc.u, Gr.u, (uint32_t)C.nx, cell, C.rxs, C.rys, g[0], -g[2] * R[21],
+                          -g[3] * R[21]);
+        if (Gr.v) acc_add(acc.v, Gr.v, (uint32_t)C.nx, cell, C.rxs, C.rys, g[1], -g[2] * R[22],
+                          -g[3] * R[22]);
+    }
+}
+
 // Vector-Jacobian product of rhs64 at (x, y, kx, ky): given the cotangent
 // (g0, g1, g2, g3) of (dx/dt, dy/dt, dkx/dt, dky/dt), adds the state
 // cotangent to (bx, by, bkx, bky) and the field-value cotangents to the

@@ -141,7 +141,10 @@ hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
 hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
 hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,
                           hipStream_t stream);
-hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, hipStream_t stream);
+// waves per 64 rays: 2 (recompute | sweep) or 4 (recompute | two linearise |
+// chain)
+hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int waves,
+                          hipStream_t stream);
 hipError_t launch_rhs64(const Rhs64Args &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_build_m64(const M64BuildArgs &a, hipStream_t stream);
 hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream);

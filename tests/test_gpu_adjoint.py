@@ -450,3 +450,40 @@ def test_adjoint_edge_cases(mr, torch):
     g0 = adjoint.adjoint(b, c, f0, 0.0, T, 1.0, 5, **vals)
     torch.cuda.synchronize()
     assert g0["x0"].numel() == 0 and float(g0["u"].abs().sum()) == 0.0
+
+
+@pytest.mark.parametrize("case", ["small", "eddy"])
+def test_two_and_four_wave_sweeps_agree(mr, torch, case):
+    """The two reverse-sweep kernels (two waves: recompute | sweep; four:
+    recompute | two linearise | chain, the default at one workgroup per CU)
+    give the same gradients: they differ only in the order of a few
+    roundings (the four-wave chain applies each stage as an explicit 4 x 4
+    matrix)."""
+    from mantaray_amd import adjoint, workloads
+    if case == "small":
+        b, c, ic, tx, ty, vals = _small_case(mr, torch)
+        T, stride = 90.0, 7
+    else:
+        cfg = workloads.eddy_adjoint(n_rays=3000, n_steps=400)
+        b, c = workloads.make_fields(cfg)
+        ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
+        vals = {"u": _dev(torch, cfg["current"]["u"].ravel()),
+                "v": _dev(torch, cfg["current"]["v"].ravel())}
+        tx, ty = cfg["target_x"], cfg["target_y"]
+        T, stride = 400.0, 50
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=stride, **vals)
+    ge = _g_end(fwd["end"], tx, ty, torch)
+    rows = {"kx": _dev(torch, np.random.default_rng(1).normal(size=(int(T) + 1, ic[0].numel())))}
+    g2 = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, stride, g_end=ge, g_rows=rows, block_size=128,
+                         **vals)
+    for block in (256,):
+        g4 = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, stride, g_end=ge, g_rows=rows,
+                             block_size=block, **vals)
+        torch.cuda.synchronize()
+        for k in ("x0", "y0", "kx0", "ky0", "depth", "u", "v"):
+            if g2.get(k) is None:
+                continue
+            a2, a4 = g2[k].cpu().numpy(), g4[k].cpu().numpy()
+            d = np.abs(a2 - a4).max() / max(np.abs(a2).max(), 1e-300)
+            print(f"{case} {k}: two vs {block // 64} waves {d:.2e}")
+            assert d <= 1e-12, (k, block, d)
