@@ -257,7 +257,8 @@ typedef struct mr_scene {
 typedef struct mr_batch mr_batch;
 
 /* Builds the scenes' device tables and the batch's queue and descriptors
- * (the only allocations); scenes[] is copied. */
+ * (the only allocations); scenes[] is copied, but its fields and arrays are
+ * referenced: they must outlive the batch. */
 int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *opt,
                     mr_batch **out);
 /* Enqueues the trace on opt->stream: a queue reset and one persistent
