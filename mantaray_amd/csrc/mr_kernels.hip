@@ -46,19 +46,26 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
     double sx = x, sy = y, skx = kx, sky = ky;   // stage state
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
     stop_before = false;
+    // The four stage points come in pairs at nearly one place: y + k1 dt/2
+    // and y + k2 dt/2, then y + k3 dt and the next step's start.  So the
+    // records are prefetched at stages 1 and 3 for the pair that follows
+    // (guessed at x + k dt/2 and x + k dt) and taken up at stages 2 and 4;
+    // stages 1 and 3 use the records in hand.
+    BathySlot bq = bn;
+    CurrentSlot cq = cn;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const double hs = st < 2 ? half : dt;
-        BathySlot bq = bn;
-        CurrentSlot cq = cn;
-        // the next evaluation is at x + k hs (stages 1-3) or at the
-        // next step's start, about x + k dt; only lanes whose guess
-        // leaves the cell in hand load a record
-        const double gx = x + px * hs, gy = y + py * hs;
-        if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
-            bathy_prefetch(B, BP, bn, gx, gy);
-        if (PF_C && !current_inside(cq, gx, gy))
-            current_prefetch(C, CP, cn, gx, gy);
+        if (st == 1 || st == 3) {
+            bq = bn;
+            cq = cn;
+        } else {
+            const double gx = x + px * hs, gy = y + py * hs;
+            if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
+                bathy_prefetch(B, BP, bn, gx, gy);
+            if (PF_C && !current_inside(cq, gx, gy))
+                current_prefetch(C, CP, cn, gx, gy);
+        }
         double dx, dy, dkx, dky;
         rhs<BK, CK, MATH>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
