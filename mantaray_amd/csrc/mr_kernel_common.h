@@ -29,6 +29,19 @@ __device__ __forceinline__ View load_view(const void *p)
     return v;
 }
 
+// Have a loaded view in registers at this point: each word is an input of
+// an empty asm statement, so the compiler waits for the view's loads here
+// and knows them complete in the code that follows (a loop after this does
+// not wait for them again, nor for whatever else is in flight then).
+template <class View>
+__device__ __forceinline__ void settle(const View &v)
+{
+    int buf[sizeof(View) / 4];
+    __builtin_memcpy(buf, &v, sizeof(View));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(View) / 4); ++i) __asm__ volatile("" ::"v"(buf[i]));
+}
+
 static __device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
     1.4426950408889634,                    // 1/ln 2
     6.93147180369123816490e-01,            // ln 2 split, upper part

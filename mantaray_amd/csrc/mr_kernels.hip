@@ -119,10 +119,18 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     const CurrentView C = load_view<CurrentView>(a.C);
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     const double half = a.half, dt = a.dt, sixth = a.sixth;
+    // the views and constants arrive before the loop: otherwise they count
+    // as pending at the loop head, and each step waits for every load and
+    // store in flight where it first reads them
+    settle(B);
+    settle(C);
+    settle(K);
     BathySlot bn;                      // records prefetched for the next evaluation
     CurrentSlot cn;
     if (PF_B) bathy_prefetch(B, a.bp, bn, x, y);
     if (PF_C) current_prefetch(C, a.cp, cn, x, y);
+    if (PF_B) settle(bn);              // likewise for the first records
+    if (PF_C) settle(cn);
     double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
     bool alive = true;
     int64_t len = 1;
@@ -191,6 +199,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    settle(K);                         // as in trace_kernel
     const int lane = (int)(threadIdx.x & 63);
     BathyView B{};
     CurrentView C{};
@@ -219,6 +228,8 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                     const SceneDesc *S = a.scenes + a.ray_scene[id];
                     B = load_view<BathyView>(S->B);
                     C = load_view<CurrentView>(S->C);
+                    settle(B);
+                    settle(C);
                     bp = S->bp;
                     cp = S->cp;
                     dt = S->dt; half = S->half; sixth = S->sixth;
