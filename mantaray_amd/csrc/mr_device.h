@@ -559,7 +559,52 @@ __device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, d
     }
 }
 
-// WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150)
+// The gridded queries for a target inside the slot's cell and on none of
+// its corners: the same operations as bathy_query / current_query take for
+// such a target, with no branch and no load.  slow is set where that does
+// not hold (a target outside the cell, a NaN target, an invalid slot, or a
+// corner), and those lanes must be answered by the full query.
+__device__ __forceinline__ double bathy_fast(const BathySlot &bs, double x, double y, double &hx,
+                                             double &hy, bool &slow)
+{
+    const float px = (float)x, py = (float)y;
+    const bool ok = bathy_inside(bs, px, py);
+    const float dxa = px - bs.ax.x, dya = py - bs.ay.x;
+    const float4 r0 = bs.r0, r1 = bs.r1;
+    const float hv = bilin_eval(r0.x, r0.y, r0.z, r0.w, r1.z * dya, r1.w * dxa);
+    const float dxd = px - bs.ax.y, dyb = py - bs.ay.y;
+    slow = !ok | on_corner(dxa, dxd, dya, dyb);
+    hx = (double)r1.x;
+    hy = (double)r1.y;
+    return __builtin_isnan(r1.z) ? MR_NAN : (double)hv;    // det == 0 errs
+}
+
+__device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, double y, double &u,
+                                             double &v, double &ux, double &uy, double &vxg,
+                                             double &vyg, bool &slow)
+{
+    const bool ok = current_inside(cs, x, y);
+    const float tx = (float)x, ty = (float)y;
+    const float dxa = tx - cs.xp.x, dya = ty - cs.yp.x;
+    const float uw = cs.m.x * dya;
+    const float ww = cs.m.y * dxa;
+    const float4 r0 = cs.r0, r1 = cs.r1;
+    const float uf = bilin_eval(r0.x, r0.y, r0.z, r0.w, uw, ww);
+    const float vf = bilin_eval(r1.x, r1.y, r1.z, r1.w, uw, ww);
+    const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
+    slow = !ok | on_corner(dxa, dxd, dya, dyb);
+    u = (double)uf;
+    v = (double)vf;
+    ux = cs.g0.x; uy = cs.g0.y; vxg = cs.g1.x; vyg = cs.g1.y;
+    return !__builtin_isnan(cs.m.x);                       // det == 0 errs
+}
+
+// WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150).
+// With a gridded field, every lane first evaluates on the fast path (one
+// block of straight-line code in which the compiler can interleave the two
+// interpolations with the dispersion arithmetic); the lanes it cannot
+// answer then evaluate again with the full queries, which reload records
+// and read corner values.
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
                                     const CurrentView &C, const CurrentPtrs &CP,
@@ -568,6 +613,20 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
                                     double &dky)
 {
     double hx, hy, u, v, ux, uy, vx, vy;
+    if (BK == BK_GRID || CK == CK_GRID) {
+        bool sb = false, sc = false;
+        const double h = BK == BK_GRID ? bathy_fast(bs, x, y, hx, hy, sb)
+                                       : bathy_query<BK>(B, BP, bs, x, y, hx, hy);
+        const bool okc = CK == CK_GRID ? current_fast(cs, x, y, u, v, ux, uy, vx, vy, sc)
+                                       : current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
+        odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+        if (sb | sc) {
+            const double h2 = bathy_query<BK>(B, BP, bs, x, y, hx, hy);
+            const bool okc2 = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
+            odes<MATH>(K, h2, hx, hy, okc2, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+        }
+        return;
+    }
     const double h = bathy_query<BK>(B, BP, bs, x, y, hx, hy);       // odes :120-127
     const bool okc = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);   // :130
     odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
