@@ -303,7 +303,11 @@ __device__ __forceinline__ void prefetch64(const Grid64View &B, const Grid64Ptrs
         slot64_prefetch<2>(C, CP, F.u, F.v, n.c, x, y);
 }
 
-// WaveRayPath::system in fp64 mode, with the cells held in sl.
+// WaveRayPath::system in fp64 mode, with the cells held in sl.  As rhs()
+// does for the reference precision, every lane first evaluates with the
+// cells in hand (no branch, no load: the gridded queries' own operations for
+// a target inside the slot's cell), and the lanes whose target is not inside
+// them evaluate again with the full queries, which locate and load cells.
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rhs64(const Grid64View &B, const Grid64Ptrs &BP,
                                       const Grid64View &C, const Grid64Ptrs &CP,
@@ -313,10 +317,31 @@ __device__ __forceinline__ void rhs64(const Grid64View &B, const Grid64Ptrs &BP,
 {
     Val64 h, u, v;
     Cell64 bc, cc;
-    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    bool okb, okc;
+    bool slow = false;
+    if (BK == BK_GRID) {
+        slow = !slot64_inside<1>(sl.b, x, y);
+        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1>(sl.b, x, y);
+    } else {
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    }
+    if (CK == CK_GRID) {
+        slow = slow | !slot64_inside<2>(sl.c, x, y);
+        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2>(sl.c, x, y);
+    } else {
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    }
     odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy, v.gx, v.gy,
                kx, ky, dx, dy, dkx, dky);
+    if (slow) {                        // a NaN target lands here too
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+        odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy, v.gx,
+                   v.gy, kx, ky, dx, dy, dkx, dky);
+    }
 }
 
 // ---------------------------------------------------------------------------
