@@ -416,17 +416,14 @@ __device__ __forceinline__ void unpack_cells(double v, uint32_t &b, uint32_t &c)
     c = (uint32_t)(u >> 32);
 }
 
+// The linearisation record of one evaluation from its field values (the
+// arithmetic of rhs64_lin below).
 template <int BK, int CK>
-__device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs &BP,
-                                          const Grid64View &C, const Grid64Ptrs &CP,
-                                          const Fields64 &F, Slots64 &sl, const FastConsts &K,
-                                          double x, double y, double kx, double ky,
-                                          double R[LIN_N])
+__device__ __forceinline__ void lin_record(const FastConsts &K, const Val64 &h, const Val64 &u,
+                                           const Val64 &v, const Cell64 &bc, const Cell64 &cc,
+                                           bool okb, bool okc, double kx, double ky,
+                                           double R[LIN_N])
 {
-    Val64 h, u, v;
-    Cell64 bc, cc;
-    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     const double hh = h.val;
     const double k2 = __builtin_fma(kx, kx, ky * ky);
     const double rk = rsq_nr(k2);
@@ -472,6 +469,42 @@ __device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs 
     // a state of a finite row never errs; the marks keep a misuse in bounds
     R[27] = pack_cells((BK == BK_GRID && okb) ? bc.iA : 0xFFFFFFFFu,
                        (CK == CK_GRID && okc) ? cc.iA : 0xFFFFFFFFu);
+}
+
+template <int BK, int CK>
+__device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs &BP,
+                                          const Grid64View &C, const Grid64Ptrs &CP,
+                                          const Fields64 &F, Slots64 &sl, const FastConsts &K,
+                                          double x, double y, double kx, double ky,
+                                          double R[LIN_N])
+{
+    // as rhs64: first with the cells in hand, then the full queries for the
+    // lanes whose target is not inside them
+    Val64 h, u, v;
+    Cell64 bc, cc;
+    bool okb, okc;
+    bool slow = false;
+    if (BK == BK_GRID) {
+        slow = !slot64_inside<1>(sl.b, x, y);
+        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1>(sl.b, x, y);
+    } else {
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    }
+    if (CK == CK_GRID) {
+        slow = slow | !slot64_inside<2>(sl.c, x, y);
+        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2>(sl.c, x, y);
+    } else {
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    }
+    lin_record<BK, CK>(K, h, u, v, bc, cc, okb, okc, kx, ky, R);
+    if (slow) {
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+        lin_record<BK, CK>(K, h, u, v, bc, cc, okb, okc, kx, ky, R);
+    }
 }
 
 // Applies a linearisation record R to the cotangent g: z = M g, and the
