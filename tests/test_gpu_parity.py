@@ -65,6 +65,38 @@ def test_rhs_random_states(mr, oracle_mod, seed, math):
     assert err <= (1e-12 if math == "literal" else 1e-11)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_short_traces_from_special_states(mr, oracle_mod, seed, math):
+    """One and three RK4 steps from the states of test_rhs_random_states
+    (exact nodes, cell lines, the last node row and column, points off the
+    grid, NaN and zero k).  The trace kernel answers an evaluation from the
+    cell records in hand and passes nodes, cell changes and errors to the
+    full queries; its rows match the oracle's row for row: identical row
+    counts and NaN patterns, positions within 1e-8 m, wavenumbers within
+    1e-10 of their scale."""
+    cfg = random_grid_case(seed)
+    b, c = mr.Bathymetry.from_arrays(**cfg["bathy"]), mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 4096)
+    for n in (1, 3):
+        got = mr.trace_many(b, c, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, float(n), 1.0,
+                            math=math)
+        ref = oracle_mod.trace_many(ob, oc, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0,
+                                    float(n), 1.0)
+        assert np.array_equal(got["len"], ref["len"])
+        live = int((ref["len"] == n + 1).sum())
+        assert live > 1000, live                # most rays take every step
+        for k in ("x", "y", "kx", "ky"):
+            assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), (n, k)
+        dxy = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("x", "y"))
+        kscale = np.nanmax(np.abs(ref["kx"])) + np.nanmax(np.abs(ref["ky"]))
+        dk = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("kx", "ky")) / kscale
+        print(f"seed {seed} {math} {n} steps: {live} rays live, max |d(x,y)| {dxy:.2e} m, "
+              f"max scaled |dk| {dk:.2e}")
+        assert dxy <= 1e-8 and dk <= 1e-10
+
+
 @pytest.mark.parametrize("math", ["fast", "literal"])
 def test_reference_one_step_values(mr, math):
     """wave_ray_path.rs test_axis (:405-416) and test_shallow (:521-548):
