@@ -549,10 +549,32 @@ __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs 
                                           double g3, double &bx, double &by, double &bkx,
                                           double &bky)
 {
+    // the field values as in rhs64: from the cells in hand, then the full
+    // queries for the lanes whose target is not inside them (the product and
+    // the accumulation below run once, after both)
     Val64 h, u, v;
     Cell64 bc, cc;
-    const bool okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-    const bool okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    bool okb, okc;
+    bool slow = false;
+    if (BK == BK_GRID) {
+        slow = !slot64_inside<1>(sl.b, x, y);
+        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1>(sl.b, x, y);
+    } else {
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+    }
+    if (CK == CK_GRID) {
+        slow = slow | !slot64_inside<2>(sl.c, x, y);
+        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2>(sl.c, x, y);
+    } else {
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    }
+    if (slow) {
+        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    }
     const double hh = h.val;
     const double k2 = __builtin_fma(kx, kx, ky * ky);
     const double rk = rsq_nr(k2);                       // 1/k
