@@ -6,7 +6,9 @@ compiles the HIP kernels, the NetCDF-3 reader and the C ABI with hipcc into
 mantaray_amd/libmantaray_hip.so.  Floating-point flags: -ffp-contract=off on
 host and device (no FMA contraction, as rustc never contracts), no fast-math,
 IEEE f32 division/sqrt and f32 denormals (the HIP defaults).  Packed FP32
-instructions are disabled for the device code.
+instructions are disabled for the device code.  The machine scheduler uses the
+max-ilp strategy: the integrators run one wave per SIMD, so nothing but the
+wave's own instruction-level parallelism hides its latencies.
 """
 from __future__ import annotations
 
@@ -35,6 +37,7 @@ def flags() -> list[str]:
         f"--offload-arch={ARCH}", "-mcode-object-version=5", "-O3", "-std=c++17", "-fPIC",
         "-ffp-contract=off", "-fno-fast-math",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-mllvm", "-amdgpu-sched-strategy=max-ilp",
         "-Wno-unused-command-line-argument",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     ]
