@@ -22,12 +22,12 @@ namespace mr {
 //   y <- y + (((k1 + k2 2) + k3 2) + k4) (dt/6)
 // with the stop test of WaveRayPath::solout (wave_ray_path.rs:236-246).
 //
-// Each evaluation of the right-hand side starts the loads of the cell
-// records that the NEXT evaluation will most likely use (its point guessed
-// from the latest derivative) when that is not the cell already held, so a
-// record is in registers one evaluation after it was asked for and a ray
-// crossing into a new cell does not wait for memory; a wrong guess only
-// costs a blocking reload in the query.
+// Stages 1 and 3 start the loads of the cell records that the following
+// stages will most likely use (their points guessed from the latest
+// derivative) when those are not the cells already held, so a record is in
+// registers one evaluation after it was asked for and a ray crossing into a
+// new cell does not wait for memory; a wrong guess only costs a blocking
+// reload in the query (rhs, the full-query path).
 
 // One RK4 step of one lane from (x, y, kx, ky): the new state in o, and
 // stop_before when (MR_SOLOUT_NEXT_DERIV) the first stage derivative is
