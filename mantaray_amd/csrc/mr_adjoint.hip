@@ -28,9 +28,11 @@ namespace mr {
 
 // One RK4 step in fp64 mode from s: the first three stage derivatives k and
 // the new state o.  The arithmetic is that of trace_kernel's step, and so is
-// the cell prefetch: each evaluation first starts loading, into sn, the cells
-// the next evaluation will most likely need (its point guessed from the
-// latest derivative (px, py)), then evaluates with the cells in hand.
+// the cell prefetch: the stage points come in pairs at nearly one place
+// (y + k1 dt/2 with y + k2 dt/2, y + k3 dt with the next step's start), so
+// stages 1 and 3 start loading, into sn, the cells of the pair that follows
+// (guessed from the latest derivative (px, py)) and stages 2 and 4 take them
+// up; each evaluation uses the cells in hand.
 template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs &BP,
                                            const Grid64View &C, const Grid64Ptrs &CP,
@@ -41,11 +43,14 @@ __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs
 {
     double sx = s[0], sy = s[1], skx = s[2], sky = s[3];
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
+    Slots64 sq = sn;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const double hs = st < 2 ? half : dt;
-        Slots64 sq = sn;
-        prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, s[0] + px * hs, s[1] + py * hs);
+        if (st == 1 || st == 3)
+            sq = sn;
+        else
+            prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, s[0] + px * hs, s[1] + py * hs);
         double dx, dy, dkx, dky;
         rhs64<BK, CK, MATH>(B, BP, C, CP, F, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
@@ -98,6 +103,12 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
     const double half = a.half, dt = a.dt, sixth = a.sixth;
     Slots64 sn;                        // cells for the next evaluation
     prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, Slots64{}, sn, s[0], s[1]);
+    // all of it in registers before the loop, so that the loop head has
+    // nothing pending to wait for (as in trace_kernel)
+    settle(B);
+    settle(C);
+    settle(K);
+    settle(sn);
     double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
     bool alive = true;
     int64_t len = 1;
