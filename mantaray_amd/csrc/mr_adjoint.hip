@@ -510,7 +510,6 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
                     double R[LIN_N];
                     rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
-                    if (h + 1 == SPW) sn = sq;
 #pragma unroll
                     for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
                 }
