@@ -480,38 +480,76 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     P[2][q] = d[q] + d[8 + q] * half;
                     P[3][q] = d[q] + d[12 + q] * dt;
                 }
-                // this wave's stages, latest first (the wave selects values,
-                // the indices stay compile-time)
-                const int first = 3 - (wave - 1) * SPW;
-#pragma unroll
-                for (int h = 0; h < SPW; ++h) {
-                    const int st = first - h;
-                    double Y[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        Y[q] = st == 3 ? P[3][q] : st == 2 ? P[2][q] : st == 1 ? P[1][q] : P[0][q];
+                if (NL == 2) {
+                    // The stage points pair up: stages 2 and 1 lie together near
+                    // y + k dt/2, and stage 0 here lies with stage 3 of the next
+                    // round (one step earlier).  Wave 1 takes stages 2 and 1,
+                    // wave 2 stages 3 and 0, so each needs one prefetch per
+                    // round: wave 1 for the next round's midpoint (guessed one
+                    // step back along k1), wave 2 for stage 0 (known), which
+                    // it evaluates in the prefetched slot (the lanes that did
+                    // not prefetch hold the same cells in both).
                     Slots64 sq = sn;
-                    // the next point: this wave's next stage, or, after its
-                    // last, its first stage one step earlier (guessed one
-                    // step back along k1)
-                    double gx, gy;
-                    if (h + 1 < SPW) {
-                        const int sx_ = st - 1;
-                        gx = sx_ == 2 ? P[2][0] : sx_ == 1 ? P[1][0] : P[0][0];
-                        gy = sx_ == 2 ? P[2][1] : sx_ == 1 ? P[1][1] : P[0][1];
-                    } else {
-                        const double fx = first == 3 ? P[3][0] : first == 2 ? P[2][0]
-                                        : first == 1 ? P[1][0] : P[0][0];
-                        const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
-                                        : first == 1 ? P[1][1] : P[0][1];
-                        gx = fx - d[4] * dt;
-                        gy = fy - d[5] * dt;
-                    }
-                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
                     double R[LIN_N];
-                    rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
+                    if (wave == 1) {
+                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] - d[4] * dt,
+                                           P[1][1] - d[5] * dt);
+                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
+                                          P[2][2], P[2][3], R);
 #pragma unroll
-                    for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
+                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][2][f][lane] = R[f];
+                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[1][0], P[1][1],
+                                          P[1][2], P[1][3], R);
+#pragma unroll
+                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][1][f][lane] = R[f];
+                    } else {
+                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
+                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[3][0], P[3][1],
+                                          P[3][2], P[3][3], R);
+#pragma unroll
+                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][3][f][lane] = R[f];
+                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sn, K, P[0][0], P[0][1],
+                                          P[0][2], P[0][3], R);
+#pragma unroll
+                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][0][f][lane] = R[f];
+                    }
+                } else {
+                    // this wave's stages, latest first (the wave selects values,
+                    // the indices stay compile-time)
+                    const int first = 3 - (wave - 1) * SPW;
+#pragma unroll
+                    for (int h = 0; h < SPW; ++h) {
+                        const int st = first - h;
+                        double Y[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            Y[q] = st == 3   ? P[3][q]
+                                   : st == 2 ? P[2][q]
+                                   : st == 1 ? P[1][q]
+                                             : P[0][q];
+                        Slots64 sq = sn;
+                        // the next point: this wave's next stage, or, after its
+                        // last, its first stage one step earlier (guessed one
+                        // step back along k1)
+                        double gx, gy;
+                        if (h + 1 < SPW) {
+                            const int sx_ = st - 1;
+                            gx = sx_ == 2 ? P[2][0] : sx_ == 1 ? P[1][0] : P[0][0];
+                            gy = sx_ == 2 ? P[2][1] : sx_ == 1 ? P[1][1] : P[0][1];
+                        } else {
+                            const double fx = first == 3 ? P[3][0] : first == 2 ? P[2][0]
+                                            : first == 1 ? P[1][0] : P[0][0];
+                            const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
+                                            : first == 1 ? P[1][1] : P[0][1];
+                            gx = fx - d[4] * dt;
+                            gy = fy - d[5] * dt;
+                        }
+                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
+                        double R[LIN_N];
+                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
+#pragma unroll
+                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
+                    }
                 }
             } else {
                 pre = false;
