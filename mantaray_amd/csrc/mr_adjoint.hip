@@ -325,15 +325,24 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
             const int64_t i0 = (int64_t)c * stride;
             const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
             const double *wk = a.work + ((t - 1) & 1) * half_work + r;
+            // each step's state and stage derivatives are loaded one step
+            // ahead, while the step before them is swept
+            double nxt[16];
+#pragma unroll
+            for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
             for (int j = nseg - 1; j >= 0; --j) {
-                const double *w = wk + (int64_t)j * 16 * ld;
                 double k[3][4], y[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    y[q] = w[q * ld];
-                    k[0][q] = w[(4 + q) * ld];
-                    k[1][q] = w[(8 + q) * ld];
-                    k[2][q] = w[(12 + q) * ld];
+                    y[q] = nxt[q];
+                    k[0][q] = nxt[4 + q];
+                    k[1][q] = nxt[8 + q];
+                    k[2][q] = nxt[12 + q];
+                }
+                if (j >= 1) {
+                    const double *w = wk + (int64_t)(j - 1) * 16 * ld;
+#pragma unroll
+                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
                     prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, y[0] + k[2][0] * dt,
