@@ -457,14 +457,15 @@ __device__ __forceinline__ double rsq_nr(double x)
 struct FastConsts {
     double inv_ln2, ln2_hi, ln2_lo;   // 1/ln 2; ln 2 split in two parts
     double clamp;                     // |argument| limit of exp_pair
-    double inv_fact[11];              // 1/13!, 1/12!, ..., 1/3!
+    double inv_fact[9];               // 1/11!, 1/10!, ..., 1/3!
     double g, half_g;                 // g and g/2
 };
 
 // e = exp(-r) and em = 1 - exp(-r), each with small relative error, from one
 // reduced-argument evaluation: r = n ln2 + f, |f| <= ln2/2, expm1(-f) by its
-// Taylor series through degree 13 (truncation below 2^-56 relative on that
-// interval), e = 2^-n (1 + p), em = (1 - 2^-n) - 2^-n p, which has no
+// Taylor series through degree 11 (truncation below 2.2e-14 relative on
+// that interval, near the one-step Newton reciprocals it is combined with),
+// e = 2^-n (1 + p), em = (1 - 2^-n) - 2^-n p, which has no
 // cancellation for n >= 1 and is exactly -p for n = 0.  |r| is clamped to
 // 1400, beyond which exp(-r) is 0 or overflows in f64 either way.
 __device__ __forceinline__ void exp_pair(double r, double &e, double &em, const FastConsts &K)
@@ -474,10 +475,10 @@ __device__ __forceinline__ void exp_pair(double r, double &e, double &em, const 
     double f = __builtin_fma(-n, K.ln2_hi, r);
     f = __builtin_fma(-n, K.ln2_lo, f);
     const double g = -f;
-    // P(g) = sum_{j=0..12} g^j / (j+1)!,  expm1(g) = g * P(g)
+    // P(g) = sum_{j=0..10} g^j / (j+1)!,  expm1(g) = g * P(g)
     double P = K.inv_fact[0];
 #pragma unroll
-    for (int j = 1; j < 11; ++j) P = __builtin_fma(P, g, K.inv_fact[j]);
+    for (int j = 1; j < 9; ++j) P = __builtin_fma(P, g, K.inv_fact[j]);
     P = __builtin_fma(P, g, 0.5);                       // 1/2!
     P = __builtin_fma(P, g, 1.0);                       // 1/1!
     const double p = g * P;

@@ -47,8 +47,6 @@ static __device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
     6.93147180369123816490e-01,            // ln 2 split, upper part
     1.90821492927058770002e-10,            // ln 2 - upper part
     1400.0,
-    1.6059043836821614e-10,                // 1/13!
-    2.08767569878680989e-09,               // 1/12!
     2.50521083854417188e-08,               // 1/11!
     2.75573192239858907e-07,               // 1/10!
     2.75573192239858907e-06,               // 1/9!
