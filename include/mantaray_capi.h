@@ -224,7 +224,9 @@ int mr_trace_fp64(const mr_field *bathy, const mr_field *current,
  * states g_x0 .. g_ky0 [n_rays] and to the gridded values g_depth
  * [depth_len], g_u [u_len], g_v [v_len]; the grids are cleared first and
  * summed with f64 atomics, so they vary in the last bits between runs.
- * work: device buffer of mr_adjoint_workspace_size bytes. */
+ * work: device buffer of mr_adjoint_workspace_size bytes.  opt->block_size
+ * selects the sweep kernel: 0 by the launch's size, 64 / 128 / 256 one, two
+ * or four waves per 64 rays. */
 int mr_adjoint_many(const mr_field *bathy, const mr_field *current,
                     const double *depth, const double *u, const double *v,
                     int64_t n_rays, double t0, double t_end, double dt, int32_t stride,

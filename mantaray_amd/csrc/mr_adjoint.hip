@@ -370,6 +370,113 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
     }
 }
 
+// The reverse sweep on one wave per 64 rays, for launches with many more
+// waves than SIMDs.  The wave recomputes each segment, from the last, from
+// its checkpoint into the work buffer and then sweeps it back: no barrier
+// and no second buffer.  Two waves that pipeline recomputation and sweep
+// (adjoint_kernel) keep the recomputing wave about half idle, since a sweep
+// step costs about twice a forward step, and each holds a SIMD of its own;
+// once the launch has waves enough to fill every SIMD several times over,
+// one wave doing both takes about a quarter less SIMD time.
+template <int BK, int CK, int MATH>
+__global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
+{
+    const int64_t nr = a.n_rays;
+    const int64_t rr = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const bool active = rr < nr;
+    const int64_t r = active ? rr : nr - 1;
+    const Grid64View B = load_view<Grid64View>(a.B);
+    const Grid64View C = load_view<Grid64View>(a.C);
+    const FastConsts K = load_view<FastConsts>(mr_fast_consts);
+    const double dt = a.dt, half = a.half, sixth = a.sixth;
+    const int stride = a.stride;
+    const int64_t ld = a.work_ld;
+    const bool dead = all_nan4(a.end[4 * r], a.end[4 * r + 1], a.end[4 * r + 2],
+                               a.end[4 * r + 3]);
+    const int64_t m = (int64_t)a.len[r] - (dead ? 2 : 1);
+    const int cmax_r = (active && m >= 1) ? (int)((m - 1) / stride) : -1;
+    const int cmax = wave_max(cmax_r);
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active && m >= 0) {
+        if (a.g_end && !dead) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
+        }
+        const int64_t o = m * nr + r;
+        if (a.g_x) lam[0] += a.g_x[o];
+        if (a.g_y) lam[1] += a.g_y[o];
+        if (a.g_kx) lam[2] += a.g_kx[o];
+        if (a.g_ky) lam[3] += a.g_ky[o];
+    }
+    Slots64 sn;                                        // cells for the next evaluation
+    Accs64 acc;
+    double *wk = a.work + r;                           // the first half of the buffer
+    for (int c = cmax; c >= 0; --c) {
+        if (c <= cmax_r) {
+            const int64_t i0 = (int64_t)c * stride;
+            const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
+            // recompute the segment from its checkpoint
+            const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
+            double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
+            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
+            double px = 0.0, py = 0.0;
+            for (int j = 0; j < nseg; ++j) {
+                double k[3][4], n[4];
+                rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
+                                         sixth, s, k, n);
+                double *w = wk + (int64_t)j * 16 * ld;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    w[q * ld] = s[q];
+                    w[(4 + q) * ld] = k[0][q];
+                    w[(8 + q) * ld] = k[1][q];
+                    w[(12 + q) * ld] = k[2][q];
+                    s[q] = n[q];
+                }
+            }
+            // and sweep it back, each step's data loaded one step ahead
+            double nxt[16];
+#pragma unroll
+            for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
+            for (int j = nseg - 1; j >= 0; --j) {
+                double k[3][4], y[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    y[q] = nxt[q];
+                    k[0][q] = nxt[4 + q];
+                    k[1][q] = nxt[8 + q];
+                    k[2][q] = nxt[12 + q];
+                }
+                if (j >= 1) {
+                    const double *w = wk + (int64_t)(j - 1) * 16 * ld;
+#pragma unroll
+                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
+                }
+                if (j == nseg - 1)     // the sweep starts at the segment's last stage point
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, y[0] + k[2][0] * dt,
+                                       y[1] + k[2][1] * dt);
+                rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sn, a.grads, acc, K, dt, half,
+                                         sixth, y, k, lam);
+                const int64_t o = (i0 + j) * nr + r;
+                if (a.g_x) lam[0] += a.g_x[o];
+                if (a.g_y) lam[1] += a.g_y[o];
+                if (a.g_kx) lam[2] += a.g_kx[o];
+                if (a.g_ky) lam[3] += a.g_ky[o];
+            }
+        }
+    }
+    if (!active) return;
+    if (a.g_x0) a.g_x0[r] = lam[0];
+    if (a.g_y0) a.g_y0[r] = lam[1];
+    if (a.g_kx0) a.g_kx0[r] = lam[2];
+    if (a.g_ky0) a.g_ky0[r] = lam[3];
+    if (BK == BK_GRID && a.grads.depth) acc_flush(acc.h, a.grads.depth, (uint32_t)B.nx);
+    if (CK == CK_GRID) {
+        if (a.grads.u) acc_flush(acc.u, a.grads.u, (uint32_t)C.nx);
+        if (a.grads.v) acc_flush(acc.v, a.grads.v, (uint32_t)C.nx);
+    }
+}
+
 // The reverse sweep with the linearisations on waves of their own.  A
 // step's four stage linearisations (field queries, dispersion terms, the
 // matrix of rhs64_lin) do not depend on the cotangent chain, only their
@@ -681,16 +788,19 @@ hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int wa
 {
     if (a.n_rays <= 0) return hipSuccess;
     if (a.stride <= 0 || a.work_ld < a.n_rays) return hipErrorInvalidValue;
-    if (waves != 2 && waves != 4) return hipErrorInvalidValue;
+    if (waves != 1 && waves != 2 && waves != 4) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((a.n_rays + 63) / 64));    // 64 rays per workgroup
     dispatch_kinds(bk, ck, math, [&](auto k) {
         using KK = decltype(k);
         if (waves == 4)
             hipLaunchKernelGGL((adjoint_lin_kernel<KK::bk, KK::ck, KK::math, 2>), grid,
                                dim3(4 * 64), 0, stream, a);
-        else
+        else if (waves == 2)
             hipLaunchKernelGGL((adjoint_kernel<KK::bk, KK::ck, KK::math>), grid,
                                dim3(ADJ_BLOCK), 0, stream, a);
+        else
+            hipLaunchKernelGGL((adjoint_seq_kernel<KK::bk, KK::ck, KK::math>), grid, dim3(64),
+                               0, stream, a);
     });
     return hipGetLastError();
 }

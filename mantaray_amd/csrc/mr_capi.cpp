@@ -1244,19 +1244,24 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
     a.grads.depth = g_depth;
     a.grads.u = g_u;
     a.grads.v = g_v;
-    // four waves per 64 rays while the launch has at most one workgroup per
-    // CU (latency-bound), two beyond; block_size 128 or 256 forces two or four
+    // the sweep kernel by workgroups (64 rays each) per CU: four waves per
+    // workgroup up to one per CU (latency-bound: recompute | two linearise |
+    // chain), two up to two per CU (recompute | sweep), one beyond (each wave
+    // recomputes and sweeps in turn, the least SIMD time once the launch fills
+    // the GPU); block_size 64, 128 or 256 forces one, two or four
     int cus = 0;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    int waves = (n_rays + 63) / 64 <= cus ? 4 : 2;
-    if (o.block_size == 128 || o.block_size == 256)
+    const int64_t groups = (n_rays + 63) / 64;
+    int waves = groups <= cus ? 4 : groups <= 2 * (int64_t)cus ? 2 : 1;
+    if (o.block_size == 64 || o.block_size == 128 || o.block_size == 256)
         waves = o.block_size / 64;
     else if (o.block_size != 0)
-        return fail(MR_ERR_INVALID_ARGUMENT, "mr_adjoint_many: block_size must be 0, 128 or 256");
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "mr_adjoint_many: block_size must be 0, 64, 128 or 256");
     MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, waves,
                               stream));
     return MR_OK;

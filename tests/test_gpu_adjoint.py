@@ -453,12 +453,13 @@ def test_adjoint_edge_cases(mr, torch):
 
 
 @pytest.mark.parametrize("case", ["small", "eddy"])
-def test_two_and_four_wave_sweeps_agree(mr, torch, case):
-    """The two reverse-sweep kernels (two waves: recompute | sweep; four:
-    recompute | two linearise | chain, the default at one workgroup per CU)
-    give the same gradients: they differ only in the order of a few
-    roundings (the four-wave chain applies each stage as an explicit 4 x 4
-    matrix)."""
+def test_sweep_kernels_agree(mr, torch, case):
+    """The three reverse-sweep kernels (two waves: recompute | sweep; four:
+    recompute | two linearise | chain, the default at one workgroup per CU;
+    one wave recomputing and sweeping each segment in turn) give the same
+    gradients: they differ only in the order of a few roundings (the
+    four-wave chain applies each stage as an explicit 4 x 4 matrix; the
+    order of concurrent atomic additions to the grids)."""
     from mantaray_amd import adjoint, workloads
     if case == "small":
         b, c, ic, tx, ty, vals = _small_case(mr, torch)
@@ -476,7 +477,7 @@ def test_two_and_four_wave_sweeps_agree(mr, torch, case):
     rows = {"kx": _dev(torch, np.random.default_rng(1).normal(size=(int(T) + 1, ic[0].numel())))}
     g2 = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, stride, g_end=ge, g_rows=rows, block_size=128,
                          **vals)
-    for block in (256,):
+    for block in (256, 64):
         g4 = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, stride, g_end=ge, g_rows=rows,
                              block_size=block, **vals)
         torch.cuda.synchronize()
