@@ -203,6 +203,7 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
         Y[st][0] = s[0] + k[st - 1][0] * hs;
         Y[st][1] = s[1] + k[st - 1][1] * hs;
     }
+    Slots64 sq = sn;
 #pragma unroll
     for (int st = 3; st >= 0; --st) {
         double Yk[2];
@@ -213,15 +214,18 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
             Yk[0] = s[2] + k[st - 1][2] * hs;
             Yk[1] = s[3] + k[st - 1][3] * hs;
         }
-        // the next evaluation is at the previous stage point, known exactly;
-        // after stage 1 it is the previous step's last stage point, next to
-        // this step's start (the cell in hand)
-        Slots64 sq = sn;
-        if (st > 0) prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, Y[st - 1][0], Y[st - 1][1]);
+        // The stage points pair up: Y2 lies with Y1, and Y0 (this step's
+        // start) with the previous step's Y3.  Stages 4 and 2 (st 3, 1) start
+        // loading the cells of the pair that follows (known exactly: Y2, Y0),
+        // stages 3 and 1 (st 2, 0) take them up.
+        if (st == 2 || st == 0)
+            sq = sn;
+        else
+            prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, Y[st - 1][0], Y[st - 1][1]);
         double z[4] = {0.0, 0.0, 0.0, 0.0};
         rhs64_vjp<BK, CK>(B, BP, C, CP, F, sq, Gr, acc, K, Y[st][0], Y[st][1], Yk[0], Yk[1],
                           kb[st][0], kb[st][1], kb[st][2], kb[st][3], z[0], z[1], z[2], z[3]);
-        if (st == 0) sn = sq;
+        if (st == 0) sn = sq;          // the cells in hand, for the previous step's Y3
         if (st > 0) {
             const double hs = st < 3 ? half : dt;
 #pragma unroll
