@@ -473,8 +473,10 @@ int build64(const mr_field &f, DevTables &t)
     }
     if (f.kind == MR_BATHY_GRID || f.kind == MR_CURRENT_GRID) {
         std::vector<double> ax(f.nx), ay(f.ny);
-        for (int64_t i = 0; i < f.nx; i++) ax[(size_t)i] = bathy ? (double)f.xf[(size_t)i] : f.xd[(size_t)i];
-        for (int64_t i = 0; i < f.ny; i++) ay[(size_t)i] = bathy ? (double)f.yf[(size_t)i] : f.yd[(size_t)i];
+        for (int64_t i = 0; i < f.nx; i++)
+            ax[(size_t)i] = bathy ? (double)f.xf[(size_t)i] : f.xd[(size_t)i];
+        for (int64_t i = 0; i < f.ny; i++)
+            ay[(size_t)i] = bathy ? (double)f.yf[(size_t)i] : f.yd[(size_t)i];
         v.nx = (int)f.nx; v.ny = (int)f.ny; v.ncx = (int)(f.nx - 1);
         v.a0x = ax[0]; v.a0y = ay[0];
         v.sx = std::fabs(ax[1] - ax[0]); v.sy = std::fabs(ay[1] - ay[0]);
@@ -563,7 +565,10 @@ int resolve_options(const mr_options *opt, mr_options &o, int max_block = 256)
 }
 
 int bathy_kind_of(const mr_field *b) { return b->kind; }   // MR_BATHY_* == BK_*
-int current_kind_of(const mr_field *c) { return c->kind == MR_CURRENT_CONSTANT ? CK_CONSTANT : CK_GRID; }
+int current_kind_of(const mr_field *c)
+{
+    return c->kind == MR_CURRENT_CONSTANT ? CK_CONSTANT : CK_GRID;
+}
 
 int tuned_block(int64_t n_rays)
 {
@@ -1200,7 +1205,8 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
         return fail(MR_ERR_INVALID_ARGUMENT, "the adjoint supports MR_SOLOUT_STATE only");
     if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
     if (g_depth && bathy->kind != MR_BATHY_GRID)
-        return fail(MR_ERR_INVALID_ARGUMENT, "depth gradient requested of a non-gridded bathymetry");
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "depth gradient requested of a non-gridded bathymetry");
     if ((g_u || g_v) && current->kind != MR_CURRENT_GRID)
         return fail(MR_ERR_INVALID_ARGUMENT, "current gradient requested of a non-gridded current");
     int64_t need = 0;

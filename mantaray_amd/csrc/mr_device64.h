@@ -516,7 +516,8 @@ __device__ __forceinline__ void lin_apply(const double R[LIN_N], const Grid64Vie
 {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        z[i] = ((R[4 * i] * g[0] + R[4 * i + 1] * g[1]) + R[4 * i + 2] * g[2]) + R[4 * i + 3] * g[3];
+        z[i] = ((R[4 * i] * g[0] + R[4 * i + 1] * g[1]) + R[4 * i + 2] * g[2]) +
+               R[4 * i + 3] * g[3];
     uint32_t ib, ic;
     unpack_cells(R[27], ib, ic);
     if (BK == BK_GRID && Gr.depth && ib != 0xFFFFFFFFu) {

@@ -292,7 +292,8 @@ __global__ void __launch_bounds__(256) rhs_kernel(RhsArgs a)
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     BathySlot bs;
     CurrentSlot cs;
-    rhs<BK, CK, MATH>(B, a.bp, bs, C, a.cp, cs, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2], a.s[4 * i + 3],
+    rhs<BK, CK, MATH>(B, a.bp, bs, C, a.cp, cs, K, a.s[4 * i], a.s[4 * i + 1], a.s[4 * i + 2],
+                      a.s[4 * i + 3],
                       dx, dy, dkx, dky);
     a.ds[4 * i] = dx; a.ds[4 * i + 1] = dy; a.ds[4 * i + 2] = dkx; a.ds[4 * i + 3] = dky;
 }

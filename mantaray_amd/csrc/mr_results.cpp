@@ -176,7 +176,8 @@ int mr_ray_result_append_tsv(const char *path, const double *t, const double *x,
     FILE *f = std::fopen(path, "ab");                 // create or append
     if (!f) return mr::set_error(MR_ERR_IO, std::string("cannot open ") + path);
     const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
-    if (std::fclose(f) != 0 || !ok) return mr::set_error(MR_ERR_IO, std::string("cannot write ") + path);
+    if (std::fclose(f) != 0 || !ok)
+        return mr::set_error(MR_ERR_IO, std::string("cannot write ") + path);
     return MR_OK;
 }
 
