@@ -98,6 +98,50 @@ def test_short_traces_from_special_states(mr, oracle_mod, seed, math):
 
 
 @pytest.mark.parametrize("math", ["fast", "literal"])
+def test_truncated_value_arrays(mr, oracle_mod, math):
+    """Value arrays shorter than nx * ny, as a file's variable may be
+    (cartesian_netcdf3.rs:465-471, cartesian_current.rs:365-398): a cell
+    with a corner past the data errs, so a ray that reaches one ends with an
+    all-NaN row.  Depth, u and v are cut at different points; the states
+    start near that end of the grid and move into it.  Right-hand side and
+    trajectories against the oracle: identical NaN patterns and row counts,
+    values within the tolerances of the other tests."""
+    cfg = random_grid_case(7)
+    nx, ny = cfg["bathy"]["x"].size, cfg["bathy"]["y"].size
+    depth = cfg["bathy"]["depth"].ravel()[: nx * ny - nx - 3]
+    u = cfg["current"]["u"].ravel()[: nx * ny - 5]
+    v = cfg["current"]["v"].ravel()[: nx * ny - 2 * nx + 4]
+    b = mr.Bathymetry.from_arrays(cfg["bathy"]["x"], cfg["bathy"]["y"], depth)
+    c = mr.Current.from_arrays(cfg["current"]["x"], cfg["current"]["y"], u, v)
+    ob = oracle_mod.Bathymetry.grid(cfg["bathy"]["x"], cfg["bathy"]["y"], depth)
+    oc = oracle_mod.Current.grid(cfg["current"]["x"], cfg["current"]["y"], u, v)
+    x, y = cfg["bathy"]["x"].astype(np.float64), cfg["bathy"]["y"].astype(np.float64)
+    rng = np.random.default_rng(7)
+    n = 2048
+    th = rng.uniform(0.2, np.pi - 0.2, n)                  # heading towards +y
+    k = rng.uniform(0.02, 0.2, n)
+    s = np.stack([rng.uniform(x[0], x[-1], n), rng.uniform(y[-5], y[-1], n),
+                  k * np.cos(th), k * np.sin(th)], 1)
+    s[:64, 0] = rng.choice(x, 64)                          # on grid lines and nodes
+    s[32:96, 1] = rng.choice(y[-5:], 64)
+    ds = mr.rhs(b, c, s, math=math)
+    ref, st = oracle_mod.system_many(ob, oc, s)
+    assert np.array_equal(np.isnan(ds), np.isnan(ref))
+    assert (st != 0).sum() > 100                           # states in error cells
+    scale = np.nanmax(np.abs(ref), axis=0)
+    assert np.nanmax(np.abs(ds - ref) / scale) <= (1e-12 if math == "literal" else 1e-11)
+    got = mr.trace_many(b, c, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, 60.0, 1.0, math=math)
+    tr = oracle_mod.trace_many(ob, oc, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, 60.0, 1.0)
+    assert np.array_equal(got["len"], tr["len"])
+    ended = int((tr["len"] < 61).sum())
+    print(f"{math}: {int((st != 0).sum())} error states, {ended} of {n} rays end early")
+    assert ended > 200
+    for kk in ("x", "y", "kx", "ky"):
+        assert np.array_equal(np.isnan(got[kk]), np.isnan(tr[kk])), kk
+    assert max(np.nanmax(np.abs(got[kk] - tr[kk])) for kk in ("x", "y")) <= 1e-6
+
+
+@pytest.mark.parametrize("math", ["fast", "literal"])
 def test_reference_one_step_values(mr, math):
     """wave_ray_path.rs test_axis (:405-416) and test_shallow (:521-548):
     one RK4 step from (0, 0) with t0 = 0, tf = 1, dt = 1."""
