@@ -269,6 +269,16 @@ __device__ __forceinline__ bool bathy_inside(const BathySlot &s, float px, float
     return s.valid & (px >= s.ax.z) & (px < s.ax.w) & (py >= s.ay.z) & (py < s.ay.w);
 }
 
+// The slot's cell range alone, without its validity: for prefetch tests and
+// the branch-free evaluation, where a cell with a corner past the data
+// yields NaN all the same (its record holds NaN coefficients and slopes, so
+// the derivatives come out all-NaN, as the reference's error makes them).
+// A slot never loaded has NaN thresholds and contains no point.
+__device__ __forceinline__ bool bathy_in_cell(const BathySlot &s, float px, float py)
+{
+    return (px >= s.ax.z) & (px < s.ax.w) & (py >= s.ay.z) & (py < s.ay.w);
+}
+
 __device__ __forceinline__ void bathy_load(const BathyView &B, const BathyPtrs &P, BathySlot &s,
                                            int i1, int j1)
 {
@@ -353,6 +363,12 @@ __device__ __forceinline__ double bathy_query(const BathyView &B, const BathyPtr
 __device__ __forceinline__ bool current_inside(const CurrentSlot &s, double x, double y)
 {
     return s.valid & (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
+}
+
+// bathy_in_cell for the current's slots
+__device__ __forceinline__ bool current_in_cell(const CurrentSlot &s, double x, double y)
+{
+    return (x >= s.tx.x) & (x < s.tx.y) & (y >= s.ty.x) & (y < s.ty.y);
 }
 
 __device__ __forceinline__ void current_load(const CurrentView &C, const CurrentPtrs &P,
@@ -560,13 +576,15 @@ __device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, d
 // The gridded queries for a target inside the slot's cell and on none of
 // its corners: the same operations as bathy_query / current_query take for
 // such a target, with no branch and no load.  slow is set where that does
-// not hold (a target outside the cell, a NaN target, an invalid slot, or a
-// corner), and those lanes must be answered by the full query.
+// not hold (a target outside the cell, a NaN target, a slot never loaded, or
+// a corner), and those lanes must be answered by the full query.  In a cell
+// with a corner past the data the record's NaN values give NaN results, as
+// the full query's error does.
 __device__ __forceinline__ double bathy_fast(const BathySlot &bs, double x, double y, double &hx,
                                              double &hy, bool &slow)
 {
     const float px = (float)x, py = (float)y;
-    const bool ok = bathy_inside(bs, px, py);
+    const bool ok = bathy_in_cell(bs, px, py);
     const float dxa = px - bs.ax.x, dya = py - bs.ay.x;
     const float4 r0 = bs.r0, r1 = bs.r1;
     const float hv = bilin_eval(r0.x, r0.y, r0.z, r0.w, r1.z * dya, r1.w * dxa);
@@ -581,7 +599,7 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
                                              double &v, double &ux, double &uy, double &vxg,
                                              double &vyg, bool &slow)
 {
-    const bool ok = current_inside(cs, x, y);
+    const bool ok = current_in_cell(cs, x, y);
     const float tx = (float)x, ty = (float)y;
     const float dxa = tx - cs.xp.x, dya = ty - cs.yp.x;
     const float uw = cs.m.x * dya;

@@ -61,9 +61,9 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             cq = cn;
         } else {
             const double gx = x + px * hs, gy = y + py * hs;
-            if (PF_B && !bathy_inside(bq, (float)gx, (float)gy))
+            if (PF_B && !bathy_in_cell(bq, (float)gx, (float)gy))
                 bathy_prefetch(B, BP, bn, gx, gy);
-            if (PF_C && !current_inside(cq, gx, gy))
+            if (PF_C && !current_in_cell(cq, gx, gy))
                 current_prefetch(C, CP, cn, gx, gy);
         }
         double dx, dy, dkx, dky;
