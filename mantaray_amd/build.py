@@ -8,7 +8,11 @@ host and device (no FMA contraction, as rustc never contracts), no fast-math,
 IEEE f32 division/sqrt and f32 denormals (the HIP defaults).  Packed FP32
 instructions are disabled for the device code.  The machine scheduler uses the
 max-ilp strategy: the integrators run one wave per SIMD, so nothing but the
-wave's own instruction-level parallelism hides its latencies.
+wave's own instruction-level parallelism hides its latencies.  The post-RA
+list scheduler is turned off (-disable-post-ra): it reorders the max-ilp
+schedule after register allocation, and without it the seamount-jet forward
+trace runs about 4 % faster (10.4 against 10.85 ms per launch), the adjoint
+pair and the batch kernel 1-2 %.
 """
 from __future__ import annotations
 
@@ -37,7 +41,7 @@ def flags() -> list[str]:
         f"--offload-arch={ARCH}", "-mcode-object-version=5", "-O3", "-std=c++17", "-fPIC",
         "-ffp-contract=off", "-fno-fast-math",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-mllvm", "-amdgpu-sched-strategy=max-ilp",
+        "-mllvm", "-amdgpu-sched-strategy=max-ilp", "-mllvm", "-disable-post-ra",
         "-Wno-unused-command-line-argument",
         "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     ]
