@@ -7,7 +7,9 @@ seamount-jet case, a 1024 x 1024 grid at 200 m with a Gaussian seamount and a
 zonal jet, 65,536 rays, 5000 RK4 steps of 1 s each, full step-major
 trajectories written to HBM.  Initial conditions, field tables and output
 buffers are resident on the GPU before the timed region; the timed region is
-K back-to-back launches of the trace kernel.
+K back-to-back launches of the trace kernel.  Before it, the autotuner picks
+the record tile shape (4 x 4 to 32 x 32 cells; --tile fixes it) from two
+timed launches of each.
 
 The second half of the metric, the adjoint's cost, is measured on the
 eddy-adjoint case (512 x 512 grid at 250 m, constant depth 50 m, 20 Gaussian
@@ -23,8 +25,8 @@ time per iteration, memory and the recovered current's RMS error.
 
 The batched case (64 scenes of 256^2 to 2048^2 cells, 2^20 rays, up to
 10,000 steps each) times one persistent launch of all scenes (launch shape
-from the autotuner, and the same launch replayed from a HIP graph) against
-one launch per scene.
+and record tile shape from the autotuner, and the same launch replayed from
+a HIP graph) against one launch per scene.
 
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
@@ -61,6 +63,8 @@ def parse():
     p.add_argument("--n-steps", type=int, default=5000, help="RK4 steps per ray")
     p.add_argument("--math", default="fast", choices=["fast", "literal"])
     p.add_argument("--block", type=int, default=0)
+    p.add_argument("--tile", type=int, default=None,
+                   help="log2 of the record tile edge (default: the autotuner picks it)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--no-inversion", action="store_true", help="skip the inversion run")
@@ -244,7 +248,8 @@ def batch_cost(dev):
     """The batched workload: 64 scenes (grid sides 256-2048 at 250 m, random
     depth and eddies, 2^20 rays of uneven lifetime, steps 0.5-2 s, up to
     10,000 steps) in one persistent launch, against one launch per scene;
-    launch shape from the autotuner; the launch replayed from a HIP graph."""
+    launch shape, then record tile shape, from the autotuner (the per-scene
+    launches use the same tiles); the launch replayed from a HIP graph."""
     import numpy as np
     import torch
 
@@ -263,7 +268,7 @@ def batch_cost(dev):
     batch = Batch(scenes)
     build_s = time.perf_counter() - t0
     stream = torch.cuda.current_stream(dev)
-    best, table = batch.autotune("batched_scenes_seed2", tuner=Autotuner(
+    best, table, tile_table = batch.autotune("batched_scenes_seed2", tuner=Autotuner(
         os.path.join(ROOT, "gpurun_out", "autotune.json")), force=True)
     batch.run()
     torch.cuda.synchronize()
@@ -275,7 +280,8 @@ def batch_cost(dev):
     def per_scene():
         for sc, ic in zip(scenes, ics):
             core.trace_many_device(sc["bathymetry"], sc["current"], *ic, sc["t0"], sc["t_end"],
-                                   sc["dt"], trajectories=False, stream=stream)
+                                   sc["dt"], trajectories=False, stream=stream,
+                                   record_tile=batch.record_tile)
 
     per_scene()
     t_scenes = _event_ms(per_scene, stream, 3)
@@ -299,8 +305,9 @@ def batch_cost(dev):
             "per_scene_launches_ms": t_scenes,
             "per_scene_ray_steps_per_s": ray_steps / (t_scenes * 1e-3),
             "speedup_vs_per_scene": t_scenes / t_batch,
-            "autotuned": best,
+            "autotuned": {**best, "record_tile": batch.record_tile},
             "autotune_table_ms": [[c["block_size"], c["blocks_per_cu"], t] for c, t in table],
+            "record_tile_table_ms": [[c["record_tile"], t] for c, t in tile_table],
             "scene_generation_s": gen_s, "tables_build_s": build_s,
             "outputs": "row count and last row of every ray"}
 
@@ -333,12 +340,26 @@ def main():
         out[k] = torch.empty((n + 1, nr), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    def launch():
+    def launch(record_tile=-1):
         core.trace_many_device(b, c, *ic, cfg["t0"], cfg["t_end"], cfg["dt"], math=args.math,
-                               block_size=args.block, out=out, stream=stream)
+                               block_size=args.block, out=out, stream=stream,
+                               record_tile=record_tile)
+
+    # record tile shape: the autotuner's pick among 4 x 4 .. 32 x 32 (each
+    # candidate's tables are built in its untimed warm-up launch)
+    tile_table = []
+    if args.tile is None:
+        from mantaray_amd.autotune import Autotuner
+        best_tile, tile_table = Autotuner(os.path.join(ROOT, "gpurun_out", "autotune.json")).tune(
+            f"seamount_jet|{nr}|record_tile", [{"record_tile": t} for t in (2, 3, 4, 5)],
+            launch, reps=2, warmup=1, force=True)
+        tile = best_tile["record_tile"]
+    else:
+        tile = args.tile
+    run = lambda: launch(tile)                                   # noqa: E731
 
     for _ in range(args.warmup):
-        launch()
+        run()
     torch.cuda.synchronize()
     ray_steps = int((out["len"].to(torch.int64) - 1).sum().item())
 
@@ -350,7 +371,7 @@ def main():
     t0 = time.perf_counter()
     for e0, e1 in evs:
         e0.record(stream)
-        launch()
+        run()
         e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -378,7 +399,9 @@ def main():
         "config": {"workload": "seamount_jet", "grid": "1024x1024 @ 200 m", "rays": nr,
                    "rk4_steps": n, "dt_s": cfg["dt"], "ray_steps_per_launch": ray_steps,
                    "outputs": "full step-major trajectories (x, y, kx, ky) in HBM",
-                   "math": args.math, "parallelism": "rays, one per lane"},
+                   "math": args.math, "parallelism": "rays, one per lane",
+                   "record_tile": tile,
+                   "record_tile_table_ms": [[ct["record_tile"], t] for ct, t in tile_table]},
         "roofline": {"bound": "mfma", "roof": "FP64 (vector and matrix share the peak)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,

@@ -85,10 +85,18 @@ typedef struct mr_options {
     int32_t block_size;    /* threads per block, multiple of 64; 0 = tuned */
     int32_t blocks_per_cu; /* mr_batch_run: resident workgroups per CU; 0 = tuned */
     void   *stream;        /* hipStream_t for MR_MEM_DEVICE; NULL = null stream */
+    int32_t record_tile;   /* gridded fields, reference precision: cell records
+                              are stored in square tiles of 2^record_tile cells
+                              a side (0 = row-major ... 5 = 32 x 32); -1 =
+                              default (3, 8 x 8).  Results do not depend on it.
+                              A field keeps the tables of each tile shape it
+                              was used with until it is freed. */
 } mr_options;
 
 /* Defaults: reference precision, fast math, MR_SOLOUT_STATE, host memory,
- * current device, tuned block size. */
+ * current device, tuned block size, 8 x 8 record tiles.  A struct_size of a
+ * caller built before record_tile existed (the offset of record_tile) is
+ * accepted; record_tile then takes its default. */
 void mr_options_init(mr_options *opt);
 
 const char *mr_last_error(void);
@@ -138,8 +146,8 @@ int mr_field_grid_shape(const mr_field *f, int64_t *nx, int64_t *ny, int64_t *le
 /* Copies the axes widened to f64 and the value arrays (depth, or u then v). */
 int mr_field_copy_grid(const mr_field *f, double *x, double *y, double *a, double *b);
 
-/* Builds the field's device tables on `device` (-1 = current) now instead
- * of on first use.  No reference counterpart (the reference has no device);
+/* Builds the field's device tables (8 x 8 record tiles) on `device` (-1 =
+ * current) now instead of on first use.  No reference counterpart (the reference has no device);
  * lets callers keep table construction out of a timed region. */
 int mr_field_upload(const mr_field *f, int32_t device);
 

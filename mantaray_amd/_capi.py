@@ -72,6 +72,7 @@ class Options(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
+        ("record_tile", ctypes.c_int32),
     ]
 
 

@@ -55,7 +55,8 @@ def _opts(math="fast", solout="state", precision="reference", **kw):
 def trace_many(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
                t0: float = 0.0, t_end: float = 1.0, dt: float = 1.0, *,
                math: str = "fast", solout: str = "state", trajectories: bool = True,
-               block_size: int = 0, device: int = -1, precision: str = "reference") -> dict:
+               block_size: int = 0, device: int = -1, precision: str = "reference",
+               record_tile: int = -1) -> dict:
     """Trace rays from host arrays; returns a dict of numpy arrays.
 
     precision "reference" interpolates as the reference does (f32); "fp64"
@@ -75,7 +76,8 @@ def trace_many(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0,
     outs = [np.empty((n + 1, nr)) if trajectories else None for _ in range(4)]
     ln = np.empty(nr, dtype=np.int32)
     end = np.empty((nr, 4))
-    o = _opts(math, solout, precision, block_size=block_size, device=device)
+    o = _opts(math, solout, precision, block_size=block_size, device=device,
+              record_tile=record_tile)
     C.check(C.lib().mr_trace_many(
         bathymetry.handle, current.handle, nr, *[a.ctypes.data for a in ics],
         float(t0), float(t_end), float(dt), ctypes.byref(o), t.ctypes.data,
@@ -91,12 +93,14 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
                       t0: float = 0.0, t_end: float = 1.0, dt: float = 1.0, *,
                       math: str = "fast", solout: str = "state", trajectories: bool = True,
                       block_size: int = 0, out: dict | None = None, stream=None,
-                      precision: str = "reference") -> dict:
+                      precision: str = "reference", record_tile: int = -1) -> dict:
     """Trace rays whose initial conditions are float64 torch tensors on a GPU.
 
     Outputs are torch tensors on the same device (or the preallocated ones in
     `out`), written by kernels enqueued on `stream` (default: torch's current
-    stream); the call does not synchronise.
+    stream); the call does not synchronise.  record_tile: log2 of the edge of
+    the square tiles the gridded fields' cell records are stored in (-1 = 8 x
+    8; results do not depend on it).
     """
     import torch
 
@@ -118,7 +122,7 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
         stream = torch.cuda.current_stream(dev)
     o = _opts(math, solout, precision, block_size=block_size, memory=C.MR_MEM_DEVICE,
               device=dev.index if dev.index is not None else -1,
-              stream=ctypes.c_void_p(stream.cuda_stream))
+              stream=ctypes.c_void_p(stream.cuda_stream), record_tile=record_tile)
 
     def p(k):
         v = out.get(k)

@@ -4,7 +4,9 @@
 // Citations are to files of mines-oceanography/mantaray 0.1.0.
 #include "mantaray_capi.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -175,10 +177,13 @@ int validate_axes_f64(const std::vector<double> &x, const char *name)
 }
 
 // grids the kernels index with 32-bit cell byte offsets
-int validate_grid(int64_t nx, int64_t ny, int64_t rec_bytes)
+// record tiles along x and y for 2^tl-cell tiles
+int64_t tiles_along(int64_t n, int tl) { return (n - 1 + (int64_t(1) << tl) - 1) >> tl; }
+
+int validate_grid(int64_t nx, int64_t ny, int64_t rec_bytes, int tl)
 {
     if (nx >= 2 && ny >= 2 &&
-        ((nx - 1 + 7) / 8) * ((ny - 1 + 7) / 8) * 64 * rec_bytes >= (int64_t(1) << 32))
+        (tiles_along(nx, tl) * tiles_along(ny, tl) << (2 * tl)) * rec_bytes >= (int64_t(1) << 32))
         return fail(MR_ERR_UNSUPPORTED, "grid too large for 32-bit cell offsets");
     return MR_OK;
 }
@@ -287,7 +292,7 @@ std::vector<float2> axis_pairs(const std::vector<float> &a)
     return p;
 }
 
-int build_bathy(const mr_field &f, DevTables &t)
+int build_bathy(const mr_field &f, DevTables &t, int tl)
 {
     BathyView v;
     std::memset(&v, 0, sizeof v);
@@ -304,7 +309,7 @@ int build_bathy(const mr_field &f, DevTables &t)
         t.bp.arr = d;
         v.an = (int)f.an;
     } else {
-        int rc = validate_grid(f.nx, f.ny, 32);
+        int rc = validate_grid(f.nx, f.ny, 32, tl);
         if (!rc) rc = validate_axes_f32(f.xf, "x");
         if (!rc) rc = validate_axes_f32(f.yf, "y");
         if (rc) return rc;
@@ -332,8 +337,9 @@ int build_bathy(const mr_field &f, DevTables &t)
         if (!rc) rc = dev_upload(t, ya.data(), ya.size(), &dya);
         if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &draw);
         if (rc) return rc;
-        const int ntx = (int)((f.nx - 1 + 7) / 8), nty = (int)((f.ny - 1 + 7) / 8);
-        const int64_t ntile_cells = (int64_t)ntx * nty * 64;     // 8 x 8-cell tiles
+        const int64_t ntx = tiles_along(f.nx, tl), nty = tiles_along(f.ny, tl);
+        const int64_t ntile_cells = (ntx * nty) << (2 * tl);    // 2^tl x 2^tl-cell tiles
+        const int ntx_tl = (int)(ntx << 4 | tl);
         BCell *cells;
         rc = dev_alloc(t, (size_t)ntile_cells, &cells);
         if (rc) return rc;
@@ -343,10 +349,10 @@ int build_bathy(const mr_field &f, DevTables &t)
         ba.xspace = (double)f.xf[1] - (double)f.xf[0];   // cartesian_netcdf3.rs:119-120
         ba.yspace = (double)f.yf[1] - (double)f.yf[0];
         ba.cells = cells;
-        ba.ntx = ntx;
+        ba.ntx = ntx_tl;
         MR_HIP_TRY(launch_build_bathy(ba, 0));
         v.oob_from = (uint32_t)ba.oob_from;
-        v.ntx = ntx;
+        v.ntx = ntx_tl;
         t.bp.xa = dxa; t.bp.ya = dya; t.bp.cells = cells; t.bp.raw = draw;
     }
     BathyView *dv;
@@ -357,7 +363,7 @@ int build_bathy(const mr_field &f, DevTables &t)
     return MR_OK;
 }
 
-int build_current(const mr_field &f, DevTables &t)
+int build_current(const mr_field &f, DevTables &t, int tl)
 {
     CurrentView v;
     std::memset(&v, 0, sizeof v);
@@ -365,7 +371,7 @@ int build_current(const mr_field &f, DevTables &t)
     if (f.kind == MR_CURRENT_CONSTANT) {
         v.u = f.u; v.v = f.v;
     } else {
-        int rc = validate_grid(f.nx, f.ny, 64);
+        int rc = validate_grid(f.nx, f.ny, 64, tl);
         if (!rc) rc = validate_axes_f64(f.xd, "x");
         if (!rc) rc = validate_axes_f64(f.yd, "y");
         if (rc) return rc;
@@ -401,8 +407,9 @@ int build_current(const mr_field &f, DevTables &t)
         if (!rc) rc = dev_upload(t, f.a.data(), f.a.size(), &du);
         if (!rc) rc = dev_upload(t, f.b.data(), f.b.size(), &dv);
         if (rc) return rc;
-        const int ntx = (int)((f.nx - 1 + 7) / 8), nty = (int)((f.ny - 1 + 7) / 8);
-        const int64_t ntile_cells = (int64_t)ntx * nty * 64;     // 8 x 8-cell tiles
+        const int64_t ntx = tiles_along(f.nx, tl), nty = tiles_along(f.ny, tl);
+        const int64_t ntile_cells = (ntx * nty) << (2 * tl);    // 2^tl x 2^tl-cell tiles
+        const int ntx_tl = (int)(ntx << 4 | tl);
         CCell *cells;
         float2 *mtab;
         rc = dev_alloc(t, (size_t)ntile_cells, &cells);
@@ -416,10 +423,10 @@ int build_current(const mr_field &f, DevTables &t)
         ca.yspace = f.yd[1] - f.yd[0];
         ca.cells = cells;
         ca.mtab = mtab;
-        ca.ntx = ntx;
+        ca.ntx = ntx_tl;
         MR_HIP_TRY(launch_build_current(ca, 0));
         v.oob_from = (uint32_t)ca.oob_from;
-        v.ntx = ntx;
+        v.ntx = ntx_tl;
         t.cp.xp = dxp; t.cp.yp = dyp; t.cp.xt = dxt; t.cp.yt = dyt;
         t.cp.cells = cells; t.cp.mtab = mtab;
         t.cp.raw_u = du; t.cp.raw_v = dv;
@@ -432,23 +439,29 @@ int build_current(const mr_field &f, DevTables &t)
     return MR_OK;
 }
 
-// the device view of a field on the current device, built on first use
-int field_tables(const mr_field *cf, const DevTables **out)
+// log2 of the edge of the default record tile (8 x 8 cells)
+constexpr int kDefaultTile = 3;
+
+// the device view of a field on the current device, built on first use, per
+// record-tile shape for the gridded kinds (tl = log2 of the tile edge)
+int field_tables(const mr_field *cf, int tl, const DevTables **out)
 {
     mr_field *f = const_cast<mr_field *>(cf);
     int dev = 0;
     MR_HIP_TRY(hipGetDevice(&dev));
+    const bool gridded = f->kind == MR_BATHY_GRID || f->kind == MR_CURRENT_GRID;
+    const int key = dev * 16 + (gridded ? tl : 0);
     std::lock_guard<std::mutex> lk(f->mu);
-    auto it = f->dev.find(dev);
+    auto it = f->dev.find(key);
     if (it != f->dev.end()) {
         *out = it->second.get();
         return MR_OK;
     }
     std::unique_ptr<DevTables> t(new DevTables);
-    int rc = is_bathy(f->kind) ? build_bathy(*f, *t) : build_current(*f, *t);
+    int rc = is_bathy(f->kind) ? build_bathy(*f, *t, tl) : build_current(*f, *t, tl);
     if (rc) return rc;
     *out = t.get();
-    f->dev[dev] = std::move(t);
+    f->dev[key] = std::move(t);
     return MR_OK;
 }
 
@@ -524,7 +537,7 @@ int build64(const mr_field &f, DevTables &t)
 int field_tables64(const mr_field *cf, const DevTables **out)
 {
     const DevTables *t0;
-    int rc = field_tables(cf, &t0);
+    int rc = field_tables(cf, kDefaultTile, &t0);
     if (rc) return rc;
     mr_field *f = const_cast<mr_field *>(cf);
     std::lock_guard<std::mutex> lk(f->mu);
@@ -546,10 +559,16 @@ int resolve_options(const mr_options *opt, mr_options &o, int max_block = 256)
 {
     mr_options_init(&o);
     if (opt) {
-        if (opt->struct_size < (int32_t)sizeof(mr_options))
+        // a caller built before record_tile existed passes the shorter size
+        const int32_t old_size = (int32_t)offsetof(mr_options, record_tile);
+        if (opt->struct_size < old_size)
             return fail(MR_ERR_INVALID_ARGUMENT, "mr_options.struct_size too small");
-        o = *opt;
+        std::memcpy(&o, opt, std::min<size_t>((size_t)opt->struct_size, sizeof(mr_options)));
+        o.struct_size = (int32_t)sizeof(mr_options);
     }
+    if (o.record_tile == -1) o.record_tile = kDefaultTile;
+    if (o.record_tile < 0 || o.record_tile > 5)
+        return fail(MR_ERR_INVALID_ARGUMENT, "record_tile must be -1 or 0..5");
     if (o.precision != MR_PREC_REFERENCE && o.precision != MR_PREC_FP64_INTERP)
         return fail(MR_ERR_INVALID_ARGUMENT, "unknown precision mode");
     if (o.math != MR_MATH_FAST && o.math != MR_MATH_LITERAL)
@@ -675,8 +694,8 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                               t0, n, dt, 0, o, stream, nullptr, t_out, x_out, y_out, kx_out,
                               ky_out, len_out, end_out);
     const DevTables *bt, *ct;
-    int rc = field_tables(bathy, &bt);
-    if (!rc) rc = field_tables(current, &ct);
+    int rc = field_tables(bathy, o.record_tile, &bt);
+    if (!rc) rc = field_tables(current, o.record_tile, &ct);
     if (rc) return rc;
     if (t_out && n_rays == 0) MR_HIP_TRY(launch_fill_time(t0, dt, n, t_out, stream));
     TraceArgs a;
@@ -720,6 +739,7 @@ void mr_options_init(mr_options *opt)
     opt->block_size = 0;
     opt->blocks_per_cu = 0;
     opt->stream = nullptr;
+    opt->record_tile = -1;
 }
 
 const char *mr_last_error(void) { return g_err.c_str(); }
@@ -895,7 +915,7 @@ int mr_field_upload(const mr_field *f, int32_t device)
     DeviceGuard g(device);
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
     const DevTables *t;
-    return field_tables(f, &t);
+    return field_tables(f, kDefaultTile, &t);
 }
 
 int mr_num_steps(double t0, double t_end, double dt, int64_t *n)
@@ -1009,7 +1029,9 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
     if (o.precision == MR_PREC_FP64_INTERP) return rhs64_many(bathy, current, n, s, ds, o);
     const DevTables *bt, *ct;
-    if ((rc = field_tables(bathy, &bt)) || (rc = field_tables(current, &ct))) return rc;
+    if ((rc = field_tables(bathy, o.record_tile, &bt)) ||
+        (rc = field_tables(current, o.record_tile, &ct)))
+        return rc;
     RhsArgs a;
     a.B = bt->view;
     a.C = ct->view;
@@ -1068,7 +1090,9 @@ int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *
         int64_t n;
         if ((rc = mr_num_steps(sc.t0, sc.t_end, sc.dt, &n))) return rc;
         const DevTables *bt, *ct;
-        if ((rc = field_tables(sc.bathy, &bt)) || (rc = field_tables(sc.current, &ct))) return rc;
+        if ((rc = field_tables(sc.bathy, o.record_tile, &bt)) ||
+            (rc = field_tables(sc.current, o.record_tile, &ct)))
+            return rc;
         SceneDesc &d = desc[(size_t)i];
         d.B = bt->view; d.C = ct->view; d.bp = bt->bp; d.cp = ct->cp;
         d.dt = sc.dt; d.half = sc.dt / 2.0; d.sixth = sc.dt / 6.0;   // Rk4::new, Rk4::step

@@ -61,7 +61,7 @@ struct BathyView {
     float ax0, ay0, nxm1, nym1;           // a[0]; (f32)(n-1)
     double inv_sx, inv_sy;                // RN64(1/(double)|a1-a0|)
     uint32_t oob_from;                    // first cell (row-major) with a corner past the data
-    int ntx;                              // 8 x 8 record tiles along x
+    int ntx;                              // (record tiles along x) << 4 | log2 tile edge
     float pinv_sx, pinv_sy;               // cell guess for prefetch: (f32) inv_s
     float pmax_x, pmax_y;                 // (f32)(n - 2)
 };
@@ -80,7 +80,7 @@ struct CurrentView {
     double cx0, cy0, sx, sy, inv_sx, inv_sy;
     double nxm1, nym1;                    // n - 1
     uint32_t oob_from;                    // first cell (row-major) with a corner past u or v
-    int ntx;                              // 8 x 8 record tiles along x
+    int ntx;                              // (record tiles along x) << 4 | log2 tile edge
     float px0, py0, pinv_sx, pinv_sy;     // cell guess for prefetch, in f32
     float pmax_x, pmax_y;                 // (f32)(n - 2)
 };
@@ -151,13 +151,16 @@ __device__ __forceinline__ float2 gld2(const float2 *p, uint32_t i)
 #define MR_NAN __builtin_nan("")
 #define MR_NANF __builtin_nanf("")
 
-// Records are stored in 8 x 8-cell tiles (tile-major, row-major inside a
-// tile), so that the neighbours a ray moves into lie on the same memory page
-// as the cell it leaves.  ntx = number of tiles along x.
-__device__ __forceinline__ uint32_t tiled_cell(int i, int j, int ntx)
+// Records are stored in square tiles of 2^tl x 2^tl cells (tile-major,
+// row-major inside a tile; 8 x 8 by default), so that the neighbours a ray
+// moves into lie on the same memory page as the cell it leaves.  ntx_tl =
+// (number of tiles along x) << 4 | tl; i, j >= 0.
+__device__ __forceinline__ uint32_t tiled_cell(int i, int j, int ntx_tl)
 {
-    const uint32_t t = (uint32_t)(j >> 3) * (uint32_t)ntx + (uint32_t)(i >> 3);
-    return (t << 6) | ((uint32_t)(j & 7) << 3) | (uint32_t)(i & 7);
+    const uint32_t tl = (uint32_t)ntx_tl & 15u, ntx = (uint32_t)ntx_tl >> 4;
+    const uint32_t m = (1u << tl) - 1u;
+    const uint32_t t = ((uint32_t)j >> tl) * ntx + ((uint32_t)i >> tl);
+    return (t << (2u * tl)) | (((uint32_t)j & m) << tl) | ((uint32_t)i & m);
 }
 
 // ---------------------------------------------------------------------------

@@ -62,7 +62,7 @@ struct BathyBuildArgs {
     const double *raw;
     int64_t oob_from;                  // cells from here on are never read
     double xspace, yspace;
-    int ntx;                           // 8 x 8 record tiles along x
+    int ntx;                           // (record tiles along x) << 4 | log2 tile edge
     BCell *cells;
 };
 
@@ -72,7 +72,7 @@ struct CurrentBuildArgs {
     const double *u, *v;
     int64_t oob_from;
     double xspace, yspace;
-    int ntx;
+    int ntx;                           // as in BathyBuildArgs
     CCell *cells;
     float2 *mtab;                      // per-cell change of basis
 };

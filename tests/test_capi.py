@@ -47,6 +47,8 @@ def test_version_and_options():
     assert o.struct_size == ctypes.sizeof(C.Options)
     assert (o.precision, o.math, o.solout, o.memory, o.device, o.block_size) == \
         (C.MR_PREC_REFERENCE, C.MR_MATH_FAST, C.MR_SOLOUT_STATE, C.MR_MEM_HOST, -1, 0)
+    assert o.record_tile == -1
+    assert C.Options.record_tile.offset == 40 and ctypes.sizeof(C.Options) == 48
 
 
 def _steps(t0, t1, dt):
@@ -138,7 +140,17 @@ def test_trace_argument_errors():
     assert _trace_rc(b.handle, c.handle, 1, C.options(math=7)) == C.MR_ERR_INVALID_ARGUMENT
     assert _trace_rc(b.handle, c.handle, 1, C.options(struct_size=4)) == \
         C.MR_ERR_INVALID_ARGUMENT
+    for tile in (-2, 6):
+        assert _trace_rc(b.handle, c.handle, 1, C.options(record_tile=tile)) == \
+            C.MR_ERR_INVALID_ARGUMENT
     z = np.zeros(1)
+    # a caller built before record_tile existed passes the shorter struct:
+    # accepted, and the field past its size is not read (the bad tile here
+    # would be rejected), so the call goes on to the step-count check
+    old = C.options(struct_size=40, record_tile=6)
+    rc = L.mr_trace_many(b.handle, c.handle, 1, *([z.ctypes.data] * 4), 0.0, -5.0, 1.0,
+                         ctypes.byref(old), *([None] * 7))
+    assert rc == C.MR_ERR_STEP_COUNT
     rc = L.mr_trace_many(b.handle, c.handle, 1, *([z.ctypes.data] * 4), 0.0, -5.0, 1.0,
                          ctypes.byref(C.options()), *([None] * 7))
     assert rc == C.MR_ERR_STEP_COUNT

@@ -109,6 +109,45 @@ def test_batch_graph_replay(mr):
     batch.close()
 
 
+def test_batch_record_tiles_and_tile_tuning(mr, tmp_path):
+    """The record tile shape moves cell records, not results: batches with
+    row-major, 4 x 4 and 32 x 32 tiles give the 8 x 8 default's results
+    bitwise, and after the autotuner's tile search the batch keeps one
+    handle, of the chosen shape, with the same results."""
+    import torch
+    from mantaray_amd.autotune import Autotuner
+    from mantaray_amd.batch import Batch
+    scenes, _ = _scenes(mr, seed=7)
+    ref = Batch(scenes)
+    ref.run()
+    torch.cuda.synchronize()
+    want = [{k: v.cpu().numpy() for k, v in o.items()} for o in ref.results()]
+    ref.close()
+
+    def check(batch, what):
+        for o in batch.results():
+            o["len"].fill_(-1)
+            o["end"].fill_(-7.0)
+        batch.run()
+        torch.cuda.synchronize()
+        for i, (o, w) in enumerate(zip(batch.results(), want)):
+            assert np.array_equal(o["len"].cpu().numpy(), w["len"]), (what, i)
+            assert np.array_equal(o["end"].cpu().numpy(), w["end"], equal_nan=True), (what, i)
+
+    for tile in (0, 2, 5):
+        b = Batch(scenes, record_tile=tile)
+        assert b.record_tile == tile
+        check(b, tile)
+        b.close()
+    b = Batch(scenes)
+    best, table, tile_table = b.autotune("tiles", tuner=Autotuner(str(tmp_path / "at.json")),
+                                         force=True, tiles=(2, 3, 4))
+    assert len(table) == 9 and [c["record_tile"] for c, _ in tile_table] == [2, 3, 4]
+    assert b.record_tile == min(tile_table, key=lambda ct: ct[1])[0]["record_tile"]
+    check(b, "tuned")
+    b.close()
+
+
 def test_batch_against_oracle(mr, oracle_mod):
     import torch
     from _helpers import oracle_fields

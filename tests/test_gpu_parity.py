@@ -27,6 +27,43 @@ def mr():
     return mantaray_amd
 
 
+@pytest.fixture(scope="module")
+def jet_small():
+    from mantaray_amd import workloads
+    cfg = workloads.seamount_jet(n_rays=1024, n_steps=400)
+    b, c = workloads.make_fields(cfg)
+    return cfg, b, c
+
+
+@pytest.mark.parametrize("tile", [0, 2, 4, 5])
+def test_record_tiles_give_identical_traces(mr, jet_small, tile):
+    """The record tile shape (row-major, 4 x 4, 16 x 16, 32 x 32 cells)
+    moves cell records in memory and changes nothing else: trajectories,
+    row counts and right-hand sides are bitwise those of the 8 x 8 default
+    (seamount-jet fields, 1024 rays, 400 steps)."""
+    cfg, b, c = jet_small
+    args = (b, c, cfg["x0"], cfg["y0"], cfg["kx0"], cfg["ky0"], cfg["t0"], cfg["t_end"],
+            cfg["dt"])
+    ref = mr.trace_many(*args)
+    got = mr.trace_many(*args, record_tile=tile)
+    assert np.array_equal(got["len"], ref["len"])
+    for k in ("t", "x", "y", "kx", "ky", "end"):
+        assert np.array_equal(got[k], ref[k], equal_nan=True), k
+    s = np.stack([ref["x"][::37].ravel(), ref["y"][::37].ravel(), ref["kx"][::37].ravel(),
+                  ref["ky"][::37].ravel()], axis=1)
+    from mantaray_amd import _capi as C
+    import ctypes
+    ds = {}
+    for t in (-1, tile):
+        o = C.options(record_tile=t)
+        out = np.empty_like(s)
+        C.check(C.lib().mr_rhs_many(b.handle, c.handle, s.shape[0], s.ctypes.data,
+                                    out.ctypes.data, ctypes.byref(o)))
+        ds[t] = out
+    assert np.isfinite(ds[-1]).sum() > s.size // 2
+    assert np.array_equal(ds[tile], ds[-1], equal_nan=True)
+
+
 @pytest.mark.parametrize("math", ["fast", "literal"])
 def test_linear_beach_trajectories(mr, oracle_mod, math):
     from mantaray_amd import workloads
