@@ -682,6 +682,23 @@ int trace64_device(const mr_field *bathy, const mr_field *current, const double 
     return MR_OK;
 }
 
+// true when a gridded depth and a gridded current have the same cells: the
+// same node counts, and the current's axes rounded to f32 equal the depth's
+// (the trace then predicts both records' cells with one test; only the
+// prefetch uses this, never a result)
+bool shared_cells(const mr_field *b, const mr_field *c)
+{
+    if (b->kind != MR_BATHY_GRID || c->kind != MR_CURRENT_GRID) return false;
+    if (b->nx != c->nx || b->ny != c->ny || b->xf.size() != c->xd.size() ||
+        b->yf.size() != c->yd.size())
+        return false;
+    for (size_t i = 0; i < b->xf.size(); i++)
+        if ((float)c->xd[i] != b->xf[i]) return false;
+    for (size_t i = 0; i < b->yf.size(); i++)
+        if ((float)c->yd[i] != b->yf[i]) return false;
+    return true;
+}
+
 // the trace itself, all pointers device pointers, enqueued on stream
 int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                  const double *x0, const double *y0, const double *kx0, const double *ky0,
@@ -715,6 +732,7 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.len_out = len_out;
     a.end_out = end_out;
     a.solout = o.solout;
+    a.shared_cells = shared_cells(bathy, current) ? 1 : 0;
     const int block = o.block_size ? o.block_size : tuned_block(n_rays);
     a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,

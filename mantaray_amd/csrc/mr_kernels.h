@@ -22,6 +22,7 @@ struct TraceArgs {
     int32_t *len_out;                  // may be null
     double *end_out;                   // 4 per ray, may be null
     int solout;                        // 0 state, 1 next derivative
+    int shared_cells;                  // gridded depth and current on one set of cells
 };
 
 // One scene of a batched trace, in device memory.

@@ -33,8 +33,12 @@ namespace mr {
 // stop_before when (MR_SOLOUT_NEXT_DERIV) the first stage derivative is
 // all-NaN, i.e. the ray already ended on the row it starts from.  bn, cn:
 // the records prefetched for the next evaluation; (px, py): the latest
-// dx/dt, dy/dt, for the guess.
-template <int BK, int CK, int MATH>
+// dx/dt, dy/dt, for the guess.  SC: the depth and current grids have the
+// same cells (same node counts, current axes rounding to the depth axes), so
+// the depth record's cell test and cell guess serve both prefetches.  A
+// prefetch is only a guess (the queries check the cell they use), so this
+// changes no result.
+template <int BK, int CK, int MATH, bool SC = false>
 __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP, BathySlot &bn,
                                          const CurrentView &C, const CurrentPtrs &CP,
                                          CurrentSlot &cn, const FastConsts &K, double &px,
@@ -61,10 +65,19 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             cq = cn;
         } else {
             const double gx = x + px * hs, gy = y + py * hs;
-            if (PF_B && !bathy_in_cell(bq, (float)gx, (float)gy))
-                bathy_prefetch(B, BP, bn, gx, gy);
-            if (PF_C && !current_in_cell(cq, gx, gy))
-                current_prefetch(C, CP, cn, gx, gy);
+            if (SC) {
+                if (!bathy_in_cell(bq, (float)gx, (float)gy)) {
+                    const int i1 = guess_cell((float)gx, B.ax0, B.pinv_sx, B.pmax_x);
+                    const int j1 = guess_cell((float)gy, B.ay0, B.pinv_sy, B.pmax_y);
+                    bathy_load(B, BP, bn, i1, j1);
+                    current_load(C, CP, cn, i1, j1);
+                }
+            } else {
+                if (PF_B && !bathy_in_cell(bq, (float)gx, (float)gy))
+                    bathy_prefetch(B, BP, bn, gx, gy);
+                if (PF_C && !current_in_cell(cq, gx, gy))
+                    current_prefetch(C, CP, cn, gx, gy);
+            }
         }
         double dx, dy, dkx, dky;
         rhs<BK, CK, MATH>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
@@ -93,7 +106,7 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
 // their stores write the same bytes to the same addresses): every lane
 // stores every row, with no branch around the stores.  OUT: 0 no
 // trajectories, 1 all four components, 2 any subset.
-template <int BK, int CK, int MATH, int OUT>
+template <int BK, int CK, int MATH, int OUT, bool SC = false>
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
@@ -141,7 +154,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (alive) {
             double o[4];
             bool stop_before;
-            rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
+            rk4_step<BK, CK, MATH, SC>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
                                    a.solout, x, y, kx, ky, o, stop_before);
             if (stop_before) {
                 alive = false;
@@ -389,6 +402,15 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
     const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
     const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
                      (a.kyo != nullptr);
+    if (a.shared_cells && bk == BK_GRID && ck == CK_GRID && nout == 4) {
+        if (math == MATH_FAST)
+            hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, MATH_FAST, 1, true>), grid,
+                               dim3(block), 0, stream, a);
+        else
+            hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, MATH_LITERAL, 1, true>), grid,
+                               dim3(block), 0, stream, a);
+        return hipGetLastError();
+    }
     dispatch_kinds(bk, ck, math, [&](auto k) {
         using K = decltype(k);
         if (nout == 0)

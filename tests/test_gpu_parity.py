@@ -134,6 +134,38 @@ def test_short_traces_from_special_states(mr, oracle_mod, seed, math):
         assert dxy <= 1e-8 and dk <= 1e-10
 
 
+@pytest.mark.parametrize("shift", [0.0, 0.37, 1.0])
+def test_depth_and_current_cells(mr, oracle_mod, shift):
+    """The trace predicts the next cells with one test when the depth and
+    current grids share their cells (shift 0: current axes equal to the
+    depth's) and with a test per field otherwise (the current's axes moved
+    by 0.37 and by 1 cell).  Either way a prediction only chooses what to
+    load ahead: trajectories against the oracle, identical NaN patterns and
+    row counts, |dx|, |dy| <= 1e-6 m."""
+    cfg = random_grid_case(11)
+    bx, by = cfg["bathy"]["x"], cfg["bathy"]["y"]
+    cx = cfg["current"]["x"] + shift * float(bx[1] - bx[0])
+    cy = cfg["current"]["y"] + shift * float(by[1] - by[0])
+    b = mr.Bathymetry.from_arrays(bx, by, cfg["bathy"]["depth"])
+    c = mr.Current.from_arrays(cx, cy, cfg["current"]["u"], cfg["current"]["v"])
+    ob = oracle_mod.Bathymetry.grid(bx, by, cfg["bathy"]["depth"])
+    oc = oracle_mod.Current.grid(cx, cy, cfg["current"]["u"], cfg["current"]["v"])
+    rng = np.random.default_rng(11)
+    n = 2048
+    th = rng.uniform(0, 2 * np.pi, n)
+    k = rng.uniform(0.02, 0.2, n)
+    x, y = bx.astype(np.float64), by.astype(np.float64)
+    s = np.stack([rng.uniform(x[2], x[-3], n), rng.uniform(y[2], y[-3], n),
+                  k * np.cos(th), k * np.sin(th)], 1)
+    got = mr.trace_many(b, c, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, 80.0, 1.0)
+    tr = oracle_mod.trace_many(ob, oc, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, 80.0, 1.0)
+    assert np.array_equal(got["len"], tr["len"])
+    assert (tr["len"] > 20).sum() > n // 2
+    for kk in ("x", "y", "kx", "ky"):
+        assert np.array_equal(np.isnan(got[kk]), np.isnan(tr[kk])), kk
+    assert max(np.nanmax(np.abs(got[kk] - tr[kk])) for kk in ("x", "y")) <= 1e-6
+
+
 @pytest.mark.parametrize("math", ["fast", "literal"])
 def test_truncated_value_arrays(mr, oracle_mod, math):
     """Value arrays shorter than nx * ny, as a file's variable may be
