@@ -135,6 +135,7 @@ struct mr_field {
 struct mr_batch {
     int device = 0;
     int bk = 0, ck = 0;
+    bool shared_cells = true;           // every scene's depth and current share cells
     int64_t n_rays = 0, max_ray_steps = 0;
     std::unique_ptr<DevTables> mem;     // descriptors, ray -> scene map, queue head
     SceneDesc *scenes = nullptr;
@@ -1102,6 +1103,7 @@ int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *
         if (i == 0) { b->bk = bk; b->ck = ck; }
         else if (bk != b->bk || ck != b->ck)
             return fail(MR_ERR_INVALID_ARGUMENT, "scenes of one batch must share field kinds");
+        b->shared_cells = b->shared_cells && shared_cells(sc.bathy, sc.current);
         if (sc.n_rays < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative ray count");
         if (sc.n_rays > 0 && (!sc.x0 || !sc.y0 || !sc.kx0 || !sc.ky0 || !sc.len_out || !sc.end_out))
             return fail(MR_ERR_INVALID_ARGUMENT, "null scene arrays");
@@ -1159,6 +1161,7 @@ int mr_batch_run(mr_batch *b, const mr_options *opt)
     a.n_rays = b->n_rays;
     a.next = b->next;
     a.solout = o.solout;
+    a.shared_cells = b->shared_cells ? 1 : 0;
     MR_HIP_TRY(launch_clear_counter(b->next, stream));
     MR_HIP_TRY(launch_batch(a, b->bk, b->ck, o.math, block, blocks, stream));
     return MR_OK;

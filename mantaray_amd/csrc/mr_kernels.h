@@ -45,6 +45,7 @@ struct BatchArgs {
     int64_t n_rays;                    // over all scenes
     unsigned long long *next;          // queue head, cleared before the launch
     int solout;
+    int shared_cells;                  // as in TraceArgs, for every scene
 };
 
 struct RhsArgs {

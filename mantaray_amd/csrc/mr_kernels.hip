@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 // atomic add per wave.  Each lane holds its scene's field views, pointers and
 // step in registers, and its record slots are emptied when it changes rays.
 // Outputs: each ray's row count and last row, as trace_kernel writes them.
-template <int BK, int CK, int MATH>
+template <int BK, int CK, int MATH, bool SC = false>
 __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
             if (!finished) {
                 double o[4];
                 bool stop_before;
-                rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
+                rk4_step<BK, CK, MATH, SC>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
                                        a.solout, x, y, kx, ky, o, stop_before);
                 ++step;
                 if (stop_before) {
@@ -431,6 +431,15 @@ hipError_t launch_batch(const BatchArgs &a, int bk, int ck, int math, int block,
 {
     if (a.n_rays <= 0) return hipSuccess;
     if (block < 64 || block > 256 || block % 64 || blocks < 1) return hipErrorInvalidValue;
+    if (a.shared_cells && bk == BK_GRID && ck == CK_GRID) {
+        if (math == MATH_FAST)
+            hipLaunchKernelGGL((batch_kernel<BK_GRID, CK_GRID, MATH_FAST, true>),
+                               dim3((unsigned)blocks), dim3(block), 0, stream, a);
+        else
+            hipLaunchKernelGGL((batch_kernel<BK_GRID, CK_GRID, MATH_LITERAL, true>),
+                               dim3((unsigned)blocks), dim3(block), 0, stream, a);
+        return hipGetLastError();
+    }
     dispatch_kinds(bk, ck, math, [&](auto k) {
         using KK = decltype(k);
         hipLaunchKernelGGL((batch_kernel<KK::bk, KK::ck, KK::math>), dim3((unsigned)blocks),

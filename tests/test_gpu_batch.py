@@ -16,7 +16,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _scenes(mr, seed=5):
+def _scenes(mr, seed=5, shift=0.0):
+    """shift: the current's axes moved by that many cells from the depth's
+    (0: the two fields share their cells)."""
     from mantaray_amd import workloads as W
     rs = W.Stream(seed)
     specs = [(64, 300, 700, 1.0), (128, 0, 200, 0.5), (96, 1500, 900, 2.0),
@@ -32,7 +34,7 @@ def _scenes(mr, seed=5):
         th = rs.uniform(nr, 0.0, 2 * np.pi)
         k = W.wavenumber(2 * np.pi / rs.uniform(nr, 6.0, 16.0), np.full(nr, 2000.0))
         cfg = {"bathy": {"x": xd.astype(np.float32), "y": xd.astype(np.float32), "depth": depth},
-               "current": {"x": xd, "y": xd.copy(), "u": u, "v": v},
+               "current": {"x": xd + shift * dx, "y": xd + shift * dx, "u": u, "v": v},
                "x0": x0, "y0": y0, "kx0": k * np.cos(th), "ky0": k * np.sin(th),
                "t0": 0.0, "t_end": steps * dt, "dt": dt}
         b, c = W.make_fields(cfg)
@@ -148,11 +150,14 @@ def test_batch_record_tiles_and_tile_tuning(mr, tmp_path):
     b.close()
 
 
-def test_batch_against_oracle(mr, oracle_mod):
+@pytest.mark.parametrize("shift", [0.0, 0.5])
+def test_batch_against_oracle(mr, oracle_mod, shift):
+    """Endpoints against the oracle, with the depth and current on the same
+    cells (one prefetch test for both) and on cells half a cell apart."""
     import torch
     from _helpers import oracle_fields
     from mantaray_amd.batch import Batch
-    scenes, cfgs = _scenes(mr, seed=7)
+    scenes, cfgs = _scenes(mr, seed=7, shift=shift)
     batch = Batch(scenes)
     batch.run()
     torch.cuda.synchronize()
