@@ -598,6 +598,15 @@ __device__ __forceinline__ double bathy_fast(const BathySlot &bs, double x, doub
     return __builtin_isnan(r1.z) ? MR_NAN : (double)hv;    // det == 0 errs
 }
 
+//
+// SC (the depth and current grids share their cells, i.e. the current's
+// axes rounded to f32 are the depth's): the current's corner test is left
+// out, because whenever both cell tests pass it equals the depth's.  The two
+// slots then hold the same cell or, for a target within rounding distance of
+// a node line, the two cells beside that line; their corner sets along that
+// axis share the node on it, and the other corners lie a cell away from the
+// target.  Along the other axis the same holds.
+template <bool SC = false>
 __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, double y, double &u,
                                              double &v, double &ux, double &uy, double &vxg,
                                              double &vyg, bool &slow)
@@ -610,8 +619,12 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
     const float4 r0 = cs.r0, r1 = cs.r1;
     const float uf = bilin_eval(r0.x, r0.y, r0.z, r0.w, uw, ww);
     const float vf = bilin_eval(r1.x, r1.y, r1.z, r1.w, uw, ww);
-    const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
-    slow = !ok | on_corner(dxa, dxd, dya, dyb);
+    if (SC) {
+        slow = !ok;
+    } else {
+        const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
+        slow = !ok | on_corner(dxa, dxd, dya, dyb);
+    }
     u = (double)uf;
     v = (double)vf;
     ux = cs.g0.x; uy = cs.g0.y; vxg = cs.g1.x; vyg = cs.g1.y;
@@ -624,7 +637,7 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
 // interpolations with the dispersion arithmetic); the lanes it cannot
 // answer then evaluate again with the full queries, which reload records
 // and read corner values.
-template <int BK, int CK, int MATH>
+template <int BK, int CK, int MATH, bool SC = false>
 __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
                                     const CurrentView &C, const CurrentPtrs &CP,
                                     CurrentSlot &cs, const FastConsts &K, double x, double y,
@@ -636,7 +649,7 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
         bool sb = false, sc = false;
         const double h = BK == BK_GRID ? bathy_fast(bs, x, y, hx, hy, sb)
                                        : bathy_query<BK>(B, BP, bs, x, y, hx, hy);
-        const bool okc = CK == CK_GRID ? current_fast(cs, x, y, u, v, ux, uy, vx, vy, sc)
+        const bool okc = CK == CK_GRID ? current_fast<SC>(cs, x, y, u, v, ux, uy, vx, vy, sc)
                                        : current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
         odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
         if (sb | sc) {

@@ -35,9 +35,9 @@ namespace mr {
 // the records prefetched for the next evaluation; (px, py): the latest
 // dx/dt, dy/dt, for the guess.  SC: the depth and current grids have the
 // same cells (same node counts, current axes rounding to the depth axes), so
-// the depth record's cell test and cell guess serve both prefetches.  A
-// prefetch is only a guess (the queries check the cell they use), so this
-// changes no result.
+// the depth record's cell test and cell guess serve both prefetches (a
+// prefetch is only a guess: the queries check the cell they use), and the
+// depth's corner test serves both fields (current_fast).  No result changes.
 template <int BK, int CK, int MATH, bool SC = false>
 __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP, BathySlot &bn,
                                          const CurrentView &C, const CurrentPtrs &CP,
@@ -80,7 +80,7 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             }
         }
         double dx, dy, dkx, dky;
-        rhs<BK, CK, MATH>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        rhs<BK, CK, MATH, SC>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
             // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
