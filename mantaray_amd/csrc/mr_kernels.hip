@@ -402,13 +402,21 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
     const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
     const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
                      (a.kyo != nullptr);
-    if (a.shared_cells && bk == BK_GRID && ck == CK_GRID && nout == 4) {
-        if (math == MATH_FAST)
-            hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, MATH_FAST, 1, true>), grid,
-                               dim3(block), 0, stream, a);
-        else
-            hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, MATH_LITERAL, 1, true>), grid,
-                               dim3(block), 0, stream, a);
+    if (a.shared_cells && bk == BK_GRID && ck == CK_GRID) {
+        auto go = [&](auto m) {
+            constexpr int M = decltype(m)::value;
+            if (nout == 0)
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 0, true>), grid,
+                                   dim3(block), 0, stream, a);
+            else if (nout == 4)
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 1, true>), grid,
+                                   dim3(block), 0, stream, a);
+            else
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 2, true>), grid,
+                                   dim3(block), 0, stream, a);
+        };
+        if (math == MATH_FAST) go(std::integral_constant<int, MATH_FAST>{});
+        else go(std::integral_constant<int, MATH_LITERAL>{});
         return hipGetLastError();
     }
     dispatch_kinds(bk, ck, math, [&](auto k) {
