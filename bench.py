@@ -182,8 +182,7 @@ def adjoint_cost(dev, reps=5):
     ray_steps = int((f["len"].to(torch.int64) - 1).sum().item())
     from mantaray_amd.autotune import Autotuner
     best, table = adjoint.tune_stride(b, c, *ic, 0.0, T, dt, u=u, v=v, force=True,
-                                      tuner=Autotuner(os.path.join(ROOT, "gpurun_out",
-                                                                   "autotune.json")))
+                                      tuner=Autotuner())
     return {"workload": "eddy_adjoint", "grid": "512x512 @ 250 m", "rays": nr,
             "rk4_steps": int(T / dt), "checkpoint_stride": stride,
             "loss": "sum_r |(x, y)_r(T) - target_r|^2",
@@ -268,8 +267,8 @@ def batch_cost(dev):
     batch = Batch(scenes)
     build_s = time.perf_counter() - t0
     stream = torch.cuda.current_stream(dev)
-    best, table, tile_table = batch.autotune("batched_scenes_seed2", tuner=Autotuner(
-        os.path.join(ROOT, "gpurun_out", "autotune.json")), force=True)
+    best, table, tile_table = batch.autotune("batched_scenes_seed2", tuner=Autotuner(),
+                                             force=True)
     batch.run()
     torch.cuda.synchronize()
     ray_steps = int(sum(int((o["len"].to(torch.int64) - 1).sum()) for o in batch.results()))
@@ -350,7 +349,7 @@ def main():
     tile_table = []
     if args.tile is None:
         from mantaray_amd.autotune import Autotuner
-        best_tile, tile_table = Autotuner(os.path.join(ROOT, "gpurun_out", "autotune.json")).tune(
+        best_tile, tile_table = Autotuner().tune(
             f"seamount_jet|{nr}|record_tile", [{"record_tile": t} for t in (2, 3, 4, 5)],
             launch, reps=2, warmup=1, force=True)
         tile = best_tile["record_tile"]
