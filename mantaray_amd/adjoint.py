@@ -50,12 +50,14 @@ def workspace_bytes(n_rays: int, stride: int) -> int:
     return n.value
 
 
-def _check_values(field, arr, which, length):
+def _check_values(field, arr, which, length, device=None):
     import torch
     if arr is None:
         return None
     if not isinstance(arr, torch.Tensor) or arr.dtype != torch.float64 or not arr.is_cuda:
         raise TypeError(f"{which} must be a float64 CUDA tensor")
+    if device is not None and arr.device != device:
+        raise ValueError(f"{which} is on {arr.device}; the trace runs on {device}")
     if arr.numel() != length:
         raise ValueError(f"{which} has {arr.numel()} values; the field has {length}")
     return arr.contiguous()
@@ -83,17 +85,16 @@ def trace_fp64(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0, t0: f
     (n_ckpt, 4, n_rays), the state every stride steps.
     """
     import torch
-    dev = x0.device
+
+    from ._tensors import check_initial_conditions
+    dev = check_initial_conditions((x0, y0, kx0, ky0))
     ics = [a.contiguous() for a in (x0, y0, kx0, ky0)]
-    for a in ics:
-        if a.dtype != torch.float64 or a.device != dev:
-            raise TypeError("initial conditions must be float64 tensors on one device")
     nr = min(a.numel() for a in ics)
     la, _ = value_lengths(bathymetry)
     lu, lv = value_lengths(current)
-    depth = _check_values(bathymetry, depth, "depth", la)
-    u = _check_values(current, u, "u", lu)
-    v = _check_values(current, v, "v", lv)
+    depth = _check_values(bathymetry, depth, "depth", la, dev)
+    u = _check_values(current, u, "u", lu, dev)
+    v = _check_values(current, v, "v", lv, dev)
     n = num_steps(t0, t_end, dt)
     f64 = dict(dtype=torch.float64, device=dev)
     out = {"len": torch.empty(nr, dtype=torch.int32, device=dev),
@@ -127,14 +128,31 @@ def adjoint(bathymetry: Bathymetry, current: Current, fwd: dict, t0: float, t_en
     u, v (the value arrays; only for gridded fields).
     """
     import torch
+
+    from ._tensors import check_device_tensor
+    if fwd.get("ckpt") is None:
+        raise ValueError("fwd holds no checkpoints: trace it with stride > 0")
     ck = fwd["ckpt"]
     dev = ck.device
     nr = fwd["len"].numel()
+    n = num_steps(t0, t_end, dt)
+    check_device_tensor(ck, "fwd['ckpt']", (checkpoint_count(t0, t_end, dt, stride), 4, nr),
+                        torch.float64, dev)
+    check_device_tensor(fwd["len"], "fwd['len']", (nr,), torch.int32, dev)
+    check_device_tensor(fwd["end"], "fwd['end']", (nr, 4), torch.float64, dev)
+    if g_end is not None:
+        check_device_tensor(g_end.contiguous(), "g_end", (nr, 4), torch.float64, dev)
+    for k, t in (g_rows or {}).items():
+        if k not in ("x", "y", "kx", "ky"):
+            raise ValueError(f"unknown row cotangent {k!r}")
+        if t is not None:
+            check_device_tensor(t.contiguous(), f"g_rows[{k!r}]", (n + 1, nr), torch.float64,
+                                dev)
     la, _ = value_lengths(bathymetry)
     lu, lv = value_lengths(current)
-    depth = _check_values(bathymetry, depth, "depth", la)
-    u = _check_values(current, u, "u", lu)
-    v = _check_values(current, v, "v", lv)
+    depth = _check_values(bathymetry, depth, "depth", la, dev)
+    u = _check_values(current, u, "u", lu, dev)
+    v = _check_values(current, v, "v", lv, dev)
     f64 = dict(dtype=torch.float64, device=dev)
     res = {}
     for k in ("x0", "y0", "kx0", "ky0"):
@@ -145,6 +163,9 @@ def adjoint(bathymetry: Bathymetry, current: Current, fwd: dict, t0: float, t_en
     res["u"] = torch.empty(lu, **f64) if "u" in want and grid_c else None
     res["v"] = torch.empty(lv, **f64) if "v" in want and grid_c else None
     nbytes = workspace_bytes(nr, stride)
+    if work is not None and (not isinstance(work, torch.Tensor) or work.device != dev or
+                             not work.is_contiguous()):
+        raise ValueError("work must be a contiguous tensor on the trace's device")
     if work is None or work.numel() * work.element_size() < nbytes:
         work = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=dev)
     rows = {k: (g_rows or {}).get(k) for k in ("x", "y", "kx", "ky")}

@@ -3,30 +3,47 @@
     python -m mantaray_amd.build
 
 compiles the HIP kernels, the NetCDF-3 reader and the C ABI with hipcc into
-mantaray_amd/libmantaray_hip.so.  Floating-point flags: -ffp-contract=off on
-host and device (no FMA contraction, as rustc never contracts), no fast-math,
-IEEE f32 division/sqrt and f32 denormals (the HIP defaults).  Packed FP32
-instructions are disabled for the device code.  The machine scheduler uses the
-max-ilp strategy: the integrators run one wave per SIMD, so nothing but the
-wave's own instruction-level parallelism hides its latencies.  The post-RA
-list scheduler is turned off (-disable-post-ra): it reorders the max-ilp
-schedule after register allocation, and without it the seamount-jet forward
-trace runs about 4 % faster (10.4 against 10.85 ms per launch), the adjoint
-pair and the batch kernel 1-2 %.
+mantaray_amd/libmantaray_hip.so (one object per source, compiled in
+parallel under mantaray_amd/_build/, then linked).  Floating-point flags:
+-ffp-contract=off on host and device (no FMA contraction, as rustc never
+contracts), no fast-math, IEEE f32 division/sqrt and f32 denormals (the HIP
+defaults).
+
+Packed FP32 instructions are disabled for the device code with the target
+feature -packed-fp32-ops.  The device pass honours it; the host (x86) pass
+does not know the feature and prints one warning per source, which is the
+only line filtered from the compiler's output.  After compiling, the build
+disassembles every object's gfx950 code object and fails if any packed FP32
+add, multiply or FMA instruction is present.
+
+The machine scheduler uses the max-ilp strategy: the integrators run one
+wave per SIMD, so nothing but the wave's own instruction-level parallelism
+hides its latencies.  The post-RA list scheduler is turned off
+(-disable-post-ra): it reorders the max-ilp schedule after register
+allocation, and without it the seamount-jet forward trace runs about 4 %
+faster (10.4 against 10.85 ms per launch), the adjoint pair and the batch
+kernel 1-2 %.
 """
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmantaray_hip.so")
 SOURCES = ["mr_kernels.hip", "mr_adjoint.hip", "mr_capi.cpp", "mr_netcdf3.cpp", "mr_results.cpp"]
 ARCH = os.environ.get("MANTARAY_OFFLOAD_ARCH", "gfx950")
+# the host pass's one message about the device-only feature
+HOST_FEATURE_WARNING = "'-packed-fp32-ops' is not a recognized feature for this target"
+# packed FP32 arithmetic the device code must not contain
+BANNED = re.compile(r"\bv_pk_(add|mul|fma)_f32\b")
 
 
 def hipcc() -> str:
@@ -34,6 +51,13 @@ def hipcc() -> str:
         if cand and os.path.exists(cand):
             return cand
     raise RuntimeError("hipcc not found")
+
+
+def llvm_tool(name: str) -> str:
+    for cand in (os.path.join("/opt/rocm/llvm/bin", name), shutil.which(name)):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError(f"{name} not found")
 
 
 def flags() -> list[str]:
@@ -47,20 +71,69 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *flags(), "-shared", "-o", tmp,
-           *[os.path.join(CSRC, s) for s in SOURCES]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _messages(stderr: str) -> list[str]:
+    return [ln for ln in stderr.splitlines()
+            if ln.strip() and HOST_FEATURE_WARNING not in ln
+            and not re.match(r"^\d+ warnings? generated", ln)]
+
+
+def _compile(src: str) -> tuple[str, list[str]]:
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc(), *flags(), "-c", "-o", obj, os.path.join(CSRC, src)]
     res = subprocess.run(cmd, capture_output=True, text=True)
-    errs = [ln for ln in res.stderr.splitlines()
-            if "unknown target feature" not in ln and "packed-fp32-ops" not in ln
-            and ln.strip() and not ln.startswith("1 warning")]
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
-    if verbose and errs:
-        print("\n".join(errs), file=sys.stderr)
+        raise RuntimeError(f"hipcc failed on {src}:\n" + "\n".join(_messages(res.stderr)))
+    return obj, _messages(res.stderr)
+
+
+def check_no_packed_fp32(obj: str) -> int:
+    """Disassemble the gfx950 code object of one object file; raise if it
+    holds a packed FP32 add, multiply or FMA.  Returns the number of
+    instructions inspected (0 for an object without device code)."""
+    fat = obj + ".fatbin"
+    co = obj + ".co"
+    try:
+        res = subprocess.run([llvm_tool("llvm-objcopy"), f"--dump-section=.hip_fatbin={fat}", obj],
+                             capture_output=True, text=True)
+        if res.returncode != 0 or not os.path.exists(fat):
+            return 0                                       # no device code in this object
+        subprocess.run([llvm_tool("clang-offload-bundler"), "--type=o", "--unbundle",
+                        f"--input={fat}", f"--output={co}",
+                        f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}"],
+                       check=True, capture_output=True, text=True)
+        dis = subprocess.run([llvm_tool("llvm-objdump"), "-d", f"--mcpu={ARCH}", co],
+                             check=True, capture_output=True, text=True).stdout
+    finally:
+        for p in (fat, co):
+            if os.path.exists(p):
+                os.remove(p)
+    bad = [ln.strip() for ln in dis.splitlines() if BANNED.search(ln)]
+    if bad:
+        raise RuntimeError(f"{os.path.basename(obj)}: packed FP32 instructions in the device "
+                           f"code ({len(bad)}), e.g. {bad[0]}")
+    return sum(1 for ln in dis.splitlines() if ln.startswith("\t"))
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    if verbose:
+        print(" ".join([hipcc(), *flags(), "-c", "<source>"]), flush=True)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        results = list(ex.map(_compile, SOURCES))
+    objs = []
+    for src, (obj, msgs) in zip(SOURCES, results):
+        objs.append(obj)
+        if verbose and msgs:
+            print(f"{src}:\n" + "\n".join(msgs), file=sys.stderr)
+    for obj in objs:
+        n = check_no_packed_fp32(obj)
+        if verbose and n:
+            print(f"{os.path.basename(obj)}: {n} device instructions, no packed FP32", flush=True)
+    tmp = LIB + ".tmp"
+    res = subprocess.run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("link failed:\n" + res.stderr)
     os.replace(tmp, LIB)
     return LIB
 

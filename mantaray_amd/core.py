@@ -104,11 +104,9 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
     """
     import torch
 
-    dev = x0.device
+    from ._tensors import check_device_tensor, check_initial_conditions
+    dev = check_initial_conditions((x0, y0, kx0, ky0))
     ics = [a.contiguous() for a in (x0, y0, kx0, ky0)]
-    for a in ics:
-        if a.dtype != torch.float64 or a.device != dev:
-            raise TypeError("initial conditions must be float64 tensors on one device")
     nr = min(a.numel() for a in ics)
     n = num_steps(t0, t_end, dt)
     if out is None:
@@ -118,6 +116,16 @@ def trace_many_device(bathymetry: Bathymetry, current: Current, x0, y0, kx0, ky0
         for k in ("x", "y", "kx", "ky"):
             out[k] = (torch.empty((n + 1, nr), dtype=torch.float64, device=dev)
                       if trajectories else None)
+    else:
+        shapes = {"t": (n + 1,), "len": (nr,), "end": (nr, 4), "x": (n + 1, nr),
+                  "y": (n + 1, nr), "kx": (n + 1, nr), "ky": (n + 1, nr)}
+        unknown = set(out) - set(shapes)
+        if unknown:
+            raise ValueError(f"unknown output names {sorted(unknown)}")
+        for k, shape in shapes.items():
+            if out.get(k) is not None:
+                check_device_tensor(out[k], f"out[{k!r}]", shape,
+                                    torch.int32 if k == "len" else torch.float64, dev)
     if stream is None:
         stream = torch.cuda.current_stream(dev)
     o = _opts(math, solout, precision, block_size=block_size, memory=C.MR_MEM_DEVICE,

@@ -3,6 +3,7 @@
 //
 // Citations are to files of mines-oceanography/mantaray 0.1.0.
 #include "mantaray_capi.h"
+#include "mr_guard.h"
 
 #include <algorithm>
 #include <cmath>
@@ -765,15 +766,18 @@ const char *mr_last_error(void) { return g_err.c_str(); }
 
 int mr_version(int32_t *major, int32_t *minor, int32_t *patch)
 {
+    MR_GUARD_BEGIN
     if (major) *major = MR_VERSION_MAJOR;
     if (minor) *minor = MR_VERSION_MINOR;
     if (patch) *patch = MR_VERSION_PATCH;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_bathy_from_arrays(int64_t nx, int64_t ny, const float *x, const float *y,
                          int64_t depth_len, const double *depth, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (nx < 0 || ny < 0 || depth_len < 0 || (nx && !x) || (ny && !y) || (depth_len && !depth))
@@ -786,12 +790,14 @@ int mr_bathy_from_arrays(int64_t nx, int64_t ny, const float *x, const float *y,
     f->a.assign(depth, depth + depth_len);
     *out = f.release();
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_current_from_arrays(int64_t nx, int64_t ny, const double *x, const double *y,
                            int64_t u_len, const double *u, int64_t v_len, const double *v,
                            mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (nx < 0 || ny < 0 || u_len < 0 || v_len < 0 || (nx && !x) || (ny && !y) ||
@@ -806,11 +812,13 @@ int mr_current_from_arrays(int64_t nx, int64_t ny, const double *x, const double
     f->b.assign(v, v + v_len);
     *out = f.release();
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_bathy_open_nc3(const char *path, const char *xname, const char *yname,
                       const char *depth_name, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out || !path || !xname || !yname || !depth_name)
         return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -826,11 +834,13 @@ int mr_bathy_open_nc3(const char *path, const char *xname, const char *yname,
         return fail(MR_ERR_FORMAT, msg);
     return mr_bathy_from_arrays((int64_t)x.size(), (int64_t)y.size(), x.data(), y.data(),
                                 (int64_t)d.size(), d.data(), out);
+    MR_GUARD_END
 }
 
 int mr_current_open_nc3(const char *path, const char *xname, const char *yname,
                         const char *uname, const char *vname, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out || !path || !xname || !yname || !uname || !vname)
         return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -845,30 +855,36 @@ int mr_current_open_nc3(const char *path, const char *xname, const char *yname,
         return fail(MR_ERR_FORMAT, msg);
     return mr_current_from_arrays((int64_t)x.size(), (int64_t)y.size(), x.data(), y.data(),
                                   (int64_t)u.size(), u.data(), (int64_t)v.size(), v.data(), out);
+    MR_GUARD_END
 }
 
 int mr_bathy_constant(float h, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     mr_field *f = new mr_field;
     f->kind = MR_BATHY_CONSTANT;
     f->h = h;
     *out = f;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_bathy_slope(float h0, float x0, float y0, float dhdx, float dhdy, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     mr_field *f = new mr_field;
     f->kind = MR_BATHY_SLOPE;
     f->h0 = h0; f->x0 = x0; f->y0 = y0; f->dhdx = dhdx; f->dhdy = dhdy;
     *out = f;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_bathy_array(int64_t n, const float *a, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (n < 0 || (n && !a)) return fail(MR_ERR_INVALID_ARGUMENT, "invalid array arguments");
@@ -878,44 +894,54 @@ int mr_bathy_array(int64_t n, const float *a, mr_field **out)
     f->arr.assign(a, a + n * n);
     *out = f;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_current_constant(double u, double v, mr_field **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     mr_field *f = new mr_field;
     f->kind = MR_CURRENT_CONSTANT;
     f->u = u; f->v = v;
     *out = f;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_field_free(mr_field *f)
 {
+    MR_GUARD_BEGIN
     delete f;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_field_kind(const mr_field *f, int32_t *kind)
 {
+    MR_GUARD_BEGIN
     if (!f || !kind) return fail(MR_ERR_INVALID_ARGUMENT, "null argument");
     *kind = f->kind;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_field_grid_shape(const mr_field *f, int64_t *nx, int64_t *ny, int64_t *len_a,
                         int64_t *len_b)
 {
+    MR_GUARD_BEGIN
     if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
     if (nx) *nx = f->nx;
     if (ny) *ny = f->ny;
     if (len_a) *len_a = (int64_t)f->a.size();
     if (len_b) *len_b = (int64_t)f->b.size();
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_field_copy_grid(const mr_field *f, double *x, double *y, double *a, double *b)
 {
+    MR_GUARD_BEGIN
     if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
     if (f->kind != MR_BATHY_GRID && f->kind != MR_CURRENT_GRID)
         return fail(MR_ERR_INVALID_ARGUMENT, "not a gridded field");
@@ -926,19 +952,23 @@ int mr_field_copy_grid(const mr_field *f, double *x, double *y, double *a, doubl
     if (a) std::memcpy(a, f->a.data(), f->a.size() * sizeof(double));
     if (b) std::memcpy(b, f->b.data(), f->b.size() * sizeof(double));
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_field_upload(const mr_field *f, int32_t device)
 {
+    MR_GUARD_BEGIN
     if (!f) return fail(MR_ERR_INVALID_ARGUMENT, "null field");
     DeviceGuard g(device);
     if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
     const DevTables *t;
     return field_tables(f, kDefaultTile, &t);
+    MR_GUARD_END
 }
 
 int mr_num_steps(double t0, double t_end, double dt, int64_t *n)
 {
+    MR_GUARD_BEGIN
     if (!n) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
     // Rk4::integrate: ((x_end - x) / step_size).ceil().to_usize().unwrap();
     // num-traits accepts -1 < q < 2^64 and truncates
@@ -948,6 +978,7 @@ int mr_num_steps(double t0, double t_end, double dt, int64_t *n)
     if (q >= 2147483647.0) return fail(MR_ERR_STEP_COUNT, "step count too large");
     *n = (int64_t)q;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_trace_many(const mr_field *bathy, const mr_field *current, int64_t n_rays,
@@ -956,6 +987,7 @@ int mr_trace_many(const mr_field *bathy, const mr_field *current, int64_t n_rays
                   double *t_out, double *x_out, double *y_out, double *kx_out,
                   double *ky_out, int32_t *len_out, double *end_out)
 {
+    MR_GUARD_BEGIN
     int rc = check_pair(bathy, current);
     if (rc) return rc;
     mr_options o;
@@ -1005,12 +1037,14 @@ int mr_trace_many(const mr_field *bathy, const mr_field *current, int64_t n_rays
     if (end_out && nr)
         MR_HIP_TRY(hipMemcpy(end_out, d_end, 4 * nr * sizeof(double), hipMemcpyDeviceToHost));
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_single_ray(const mr_field *bathy, const mr_field *current, double x0, double y0,
                   double kx0, double ky0, double t0, double t_end, double dt,
                   const mr_options *opt, double *rows_out, int64_t *n_rows)
 {
+    MR_GUARD_BEGIN
     if (!rows_out || !n_rows) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
     mr_options o;
     int rc = resolve_options(opt, o);
@@ -1034,11 +1068,13 @@ int mr_single_ray(const mr_field *bathy, const mr_field *current, double x0, dou
     }
     *n_rows = len;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const double *s,
                 double *ds, const mr_options *opt)
 {
+    MR_GUARD_BEGIN
     int rc = check_pair(bathy, current);
     if (rc) return rc;
     mr_options o;
@@ -1073,11 +1109,13 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n, const
     MR_HIP_TRY(hipDeviceSynchronize());
     if (n) MR_HIP_TRY(hipMemcpy(ds, d_ds, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *opt,
                     mr_batch **out)
 {
+    MR_GUARD_BEGIN
     if (!out) return fail(MR_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (n_scenes < 0 || (n_scenes > 0 && !scenes))
@@ -1132,10 +1170,12 @@ int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *
     MR_HIP_TRY(hipDeviceSynchronize());
     *out = b.release();
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_batch_run(mr_batch *b, const mr_options *opt)
 {
+    MR_GUARD_BEGIN
     if (!b) return fail(MR_ERR_INVALID_ARGUMENT, "null batch");
     mr_options o;
     int rc = resolve_options(opt, o);
@@ -1165,27 +1205,33 @@ int mr_batch_run(mr_batch *b, const mr_options *opt)
     MR_HIP_TRY(launch_clear_counter(b->next, stream));
     MR_HIP_TRY(launch_batch(a, b->bk, b->ck, o.math, block, blocks, stream));
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_batch_info(const mr_batch *b, int64_t *n_rays, int64_t *max_ray_steps)
 {
+    MR_GUARD_BEGIN
     if (!b) return fail(MR_ERR_INVALID_ARGUMENT, "null batch");
     if (n_rays) *n_rays = b->n_rays;
     if (max_ray_steps) *max_ray_steps = b->max_ray_steps;
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_batch_free(mr_batch *b)
 {
+    MR_GUARD_BEGIN
     if (b) {
         DeviceGuard g(b->device);
         delete b;
     }
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_checkpoint_count(double t0, double t_end, double dt, int32_t stride, int64_t *count)
 {
+    MR_GUARD_BEGIN
     if (!count) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
     if (stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "checkpoint stride must be positive");
     int64_t n;
@@ -1193,16 +1239,19 @@ int mr_checkpoint_count(double t0, double t_end, double dt, int32_t stride, int6
     if (rc) return rc;
     *count = std::max<int64_t>(1, (n + stride - 1) / stride);
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes)
 {
+    MR_GUARD_BEGIN
     if (!bytes) return fail(MR_ERR_INVALID_ARGUMENT, "null output");
     if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
     // two halves (the recompute and the reverse sweep alternate between
     // them), each stride steps x 16 doubles x n_rays
     *bytes = 2 * (int64_t)stride * 16 * n_rays * (int64_t)sizeof(double);
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_trace_fp64(const mr_field *bathy, const mr_field *current, const double *depth,
@@ -1212,6 +1261,7 @@ int mr_trace_fp64(const mr_field *bathy, const mr_field *current, const double *
                   double *t_out, double *x_out, double *y_out, double *kx_out, double *ky_out,
                   int32_t *len_out, double *end_out)
 {
+    MR_GUARD_BEGIN
     int rc = check_pair(bathy, current);
     if (rc) return rc;
     mr_options o;
@@ -1230,6 +1280,7 @@ int mr_trace_fp64(const mr_field *bathy, const mr_field *current, const double *
     return trace64_device(bathy, current, depth, u, v, n_rays, x0, y0, kx0, ky0, t0, n, dt,
                           ckpt ? stride : 0, o, (hipStream_t)o.stream, ckpt, t_out, x_out, y_out,
                           kx_out, ky_out, len_out, end_out);
+    MR_GUARD_END
 }
 
 int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double *depth,
@@ -1240,6 +1291,7 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
                     void *work, int64_t work_bytes, double *g_x0, double *g_y0, double *g_kx0,
                     double *g_ky0, double *g_depth, double *g_u, double *g_v)
 {
+    MR_GUARD_BEGIN
     int rc = check_pair(bathy, current);
     if (rc) return rc;
     mr_options o;
@@ -1316,6 +1368,7 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
     MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, waves,
                               stream));
     return MR_OK;
+    MR_GUARD_END
 }
 
 }  // extern "C"

@@ -93,19 +93,46 @@ const NcVar *find_var(const NcFile &f, const std::string &name)
     return nullptr;
 }
 
-// Element e of variable v (file order across records) as a raw pointer.
+// Offsets are formed in 128 bits, so no header value (a 64-bit begin near
+// 2^64, huge dimensions or record counts) can wrap them past the checks.
+typedef unsigned __int128 u128;
+
+// Whether every element of variable v lies inside the file, checked before
+// anything is allocated for it: the last element of a fixed-size variable,
+// and of the last record's slab of a record variable.
+bool var_fits(const NcFile &f, const NcVar &v, uint64_t &n_out)
+{
+    const u128 ts = type_size(v.type), size = f.bytes.size();
+    const u128 per = v.n_per_record;
+    if (v.is_record) {
+        const u128 n = per * (u128)f.numrecs;
+        if (n > (u128)UINT64_MAX) return false;
+        n_out = (uint64_t)n;
+        if (n == 0) return true;
+        // a slab longer than the record would overlap the next record's data
+        if (f.numrecs > 1 && per * ts > (u128)f.recsize) return false;
+        const u128 last = (u128)v.begin + (u128)(f.numrecs - 1) * (u128)f.recsize + per * ts;
+        return last <= size;
+    }
+    n_out = v.n_per_record;
+    if (per == 0) return true;
+    return (u128)v.begin + per * ts <= size;
+}
+
+// Element e of variable v (file order across records) as a raw pointer;
+// var_fits has checked that every element lies inside the file.
 const uint8_t *elem_ptr(const NcFile &f, const NcVar &v, uint64_t e)
 {
-    size_t ts = type_size(v.type);
-    uint64_t off;
+    const u128 ts = type_size(v.type);
+    u128 off;
     if (v.is_record) {
-        uint64_t rec = e / v.n_per_record, in = e % v.n_per_record;
-        off = v.begin + rec * f.recsize + in * ts;
+        const uint64_t rec = e / v.n_per_record, in = e % v.n_per_record;
+        off = (u128)v.begin + (u128)rec * f.recsize + (u128)in * ts;
     } else {
-        off = v.begin + e * ts;
+        off = (u128)v.begin + (u128)e * ts;
     }
-    if (off + ts > f.bytes.size()) return nullptr;
-    return &f.bytes[off];
+    if (off + ts > (u128)f.bytes.size()) return nullptr;
+    return &f.bytes[(size_t)off];
 }
 
 template <class Out>
@@ -116,8 +143,12 @@ int read_as(const NcFile &f, const std::string &name, std::vector<Out> &out, std
         msg = "variable '" + name + "' not found";
         return -3;
     }
-    uint64_t n = v->is_record ? v->n_per_record * f.numrecs : v->n_per_record;
-    out.resize(n);
+    uint64_t n = 0;
+    if (!var_fits(f, *v, n)) {
+        msg = "variable '" + name + "' extends past the end of the file";
+        return -3;
+    }
+    out.resize((size_t)n);
     for (uint64_t e = 0; e < n; e++) {
         const uint8_t *p = elem_ptr(f, *v, e);
         if (!p) {
@@ -203,6 +234,10 @@ int nc3_open(const std::string &path, NcFile &f, std::string &msg)
                 if (id < 0 || (size_t)id >= f.dims.size()) { msg = "bad dimension id"; return -3; }
                 uint64_t len = f.dims[(size_t)id].second;
                 if (d == 0 && len == 0) { v.is_record = true; continue; }
+                if (len != 0 && v.n_per_record > UINT64_MAX / len) {
+                    msg = "variable '" + v.name + "' has more elements than 2^64";
+                    return -3;
+                }
                 v.n_per_record *= len;
             }
             f.vars.push_back(v);
@@ -217,8 +252,14 @@ int nc3_open(const std::string &path, NcFile &f, std::string &msg)
         if (v.is_record) { n_rec_vars++; f.recsize += v.vsize; }
     }
     if (n_rec_vars == 1) {
-        for (const auto &v : f.vars)
-            if (v.is_record) f.recsize = v.n_per_record * type_size(v.type);
+        for (const auto &v : f.vars) {
+            if (!v.is_record) continue;
+            if (v.n_per_record > UINT64_MAX / type_size(v.type)) {
+                msg = "record variable '" + v.name + "' is too large";
+                return -3;
+            }
+            f.recsize = v.n_per_record * type_size(v.type);
+        }
     }
     if (streaming && f.recsize > 0) {
         uint64_t first = UINT64_MAX;

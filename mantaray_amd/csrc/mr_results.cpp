@@ -12,6 +12,7 @@
 //     and prints integral values without a fraction ("1", "-0", "NaN", "inf").
 // The digits come from std::to_chars in shortest round-trip form.
 #include "mantaray_capi.h"
+#include "mr_guard.h"
 
 #include <charconv>
 #include <cmath>
@@ -113,16 +114,13 @@ void display_number(double v, std::string &out)
 
 }  // namespace
 
-namespace mr {
-int set_error(int code, const std::string &msg);    // mr_capi.cpp
-}
-
 extern "C" {
 
 int mr_ray_result_json(const double *t, const double *x, const double *y, const double *kx,
                        const double *ky, int64_t n_rows, int64_t stride, char *buf,
                        int64_t cap, int64_t *len)
 {
+    MR_GUARD_BEGIN
     if (!len || n_rows < 0 || stride < 1 || (n_rows > 0 && (!t || !x || !y || !kx || !ky)))
         return mr::set_error(MR_ERR_INVALID_ARGUMENT, "invalid arrays");
     // RayResult::from (ray_result.rs:124-151): rows up to the first with a
@@ -151,12 +149,14 @@ int mr_ray_result_json(const double *t, const double *x, const double *y, const 
     *len = (int64_t)s.size();
     if (buf && cap > (int64_t)s.size()) std::memcpy(buf, s.c_str(), s.size() + 1);
     return MR_OK;
+    MR_GUARD_END
 }
 
 int mr_ray_result_append_tsv(const char *path, const double *t, const double *x,
                              const double *y, const double *kx, const double *ky,
                              int64_t n_rows, int64_t stride)
 {
+    MR_GUARD_BEGIN
     if (!path || n_rows < 0 || stride < 1 || (n_rows > 0 && (!t || !x || !y || !kx || !ky)))
         return mr::set_error(MR_ERR_INVALID_ARGUMENT, "invalid arrays");
     std::string s = "t x y kx ky\n";
@@ -179,6 +179,7 @@ int mr_ray_result_append_tsv(const char *path, const double *t, const double *x,
     if (std::fclose(f) != 0 || !ok)
         return mr::set_error(MR_ERR_IO, std::string("cannot write ") + path);
     return MR_OK;
+    MR_GUARD_END
 }
 
 }  // extern "C"
