@@ -228,6 +228,27 @@ def gaussian_random_field(n: int, dx: float, rs: Stream, mean: float, sd: float,
     return mean + sd * g
 
 
+def _scene_draws(rs: Stream, n_scenes: int, total_rays: int, max_steps: int):
+    """Per scene of the batched workload: grid side, ray count, step, steps."""
+    sides = np.array([256, 512, 1024, 2048])[
+        np.minimum((rs.uniform(n_scenes) * 4).astype(int), 3)]
+    raw = np.exp(rs.uniform(n_scenes, np.log(1024.0), np.log(65536.0)))
+    counts = np.maximum(1, np.floor(raw / raw.sum() * total_rays)).astype(np.int64)
+    counts[np.argmax(counts)] += total_rays - counts.sum()
+    dts = rs.uniform(n_scenes, 0.5, 2.0)
+    steps = np.floor(rs.uniform(n_scenes, 1000.0, max_steps + 1.0)).astype(np.int64)
+    return sides, counts, dts, steps
+
+
+def batched_scene_plan(n_scenes: int = 64, total_rays: int = 1 << 20, seed: int = 2,
+                       max_steps: int = 10000) -> dict:
+    """The batched workload's per-scene grid sides, ray counts, steps and
+    step counts, without building the fields (batched_scenes draws these
+    first, from the same stream)."""
+    sides, counts, dts, steps = _scene_draws(Stream(seed), n_scenes, total_rays, max_steps)
+    return {"sides": sides, "rays": counts, "dt": dts, "steps": steps}
+
+
 def batched_scenes(n_scenes: int = 64, total_rays: int = 1 << 20, seed: int = 2,
                    max_steps: int = 10000) -> list:
     """Scenes of the batched workload: grid side ~ {256, 512, 1024, 2048} at
@@ -239,13 +260,7 @@ def batched_scenes(n_scenes: int = 64, total_rays: int = 1 << 20, seed: int = 2,
     ~ U[6, 16] s and direction ~ U[0, 2 pi).  Rays leave small scenes early,
     so lifetimes are uneven."""
     rs = Stream(seed)
-    sides = np.array([256, 512, 1024, 2048])[
-        np.minimum((rs.uniform(n_scenes) * 4).astype(int), 3)]
-    raw = np.exp(rs.uniform(n_scenes, np.log(1024.0), np.log(65536.0)))
-    counts = np.maximum(1, np.floor(raw / raw.sum() * total_rays)).astype(np.int64)
-    counts[np.argmax(counts)] += total_rays - counts.sum()
-    dts = rs.uniform(n_scenes, 0.5, 2.0)
-    steps = np.floor(rs.uniform(n_scenes, 1000.0, max_steps + 1.0)).astype(np.int64)
+    sides, counts, dts, steps = _scene_draws(rs, n_scenes, total_rays, max_steps)
     dx = 250.0
     scenes = []
     for i in range(n_scenes):

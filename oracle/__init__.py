@@ -19,6 +19,9 @@ _LIB_PATH = os.path.join(_HERE, "libmr_oracle.so")
 OK, ERR, PANIC = 0, 1, 2
 PREC_REFERENCE, PREC_FP64_INTERP = 0, 1
 SOLOUT_STATE, SOLOUT_NEXT_DERIV = 0, 1
+STAGE_DIRECT, STAGE_SCALE_DT, STAGE_HALF_FIRST = 0, 1, 2
+TIME_ACCUMULATE, TIME_PRODUCT = 0, 1
+COMBO_SCALE_SIXTH, COMBO_DT_THEN_SIX = 0, 1
 BATHY_CONSTANT, BATHY_SLOPE, BATHY_ARRAY, BATHY_GRID = 0, 1, 2, 3
 CURRENT_CONSTANT, CURRENT_GRID = 0, 1
 
@@ -92,6 +95,13 @@ def lib():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
                                     _i32p, _f64p]
+        L.or_trace_many_ordered.argtypes = [ctypes.POINTER(_Bathy), ctypes.POINTER(_Current),
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                            _f64p, _f64p, _f64p, _f64p,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                            _i32p, _f64p]
         L.or_system_many.argtypes = [ctypes.POINTER(_Bathy), ctypes.POINTER(_Current),
                                      ctypes.c_int, ctypes.c_int64, _f64p, _f64p, _i32p]
         L.or_system_many.restype = None
@@ -99,7 +109,8 @@ def lib():
         L.or_fd_end_ld.argtypes = [ctypes.POINTER(_Bathy), ctypes.POINTER(_Current),
                                    ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _f64p,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                   ctypes.c_double, ctypes.c_int, _f64p, _f64p, _i32p, _i32p]
+                                   ctypes.c_double, ctypes.c_int, _f64p, _f64p, _i32p, _i32p,
+                                   _i32p]
         L.or_bathy_nearest.argtypes = [ctypes.POINTER(_Bathy), ctypes.c_float, ctypes.c_int, _f32p]
         L.or_current_nearest.argtypes = [ctypes.POINTER(_Current), ctypes.c_double, ctypes.c_int,
                                          _f64p]
@@ -291,8 +302,15 @@ def trace_single(b, c, s0, t0, t_end, dt, prec=PREC_REFERENCE, solout=SOLOUT_STA
 
 
 def trace_many(b, c, x0, y0, kx0, ky0, t0, t_end, dt, prec=PREC_REFERENCE,
-               solout=SOLOUT_STATE, nthreads=0, trajectories=True):
-    """Step-major SoA: dict with t (n+1), x/y/kx/ky (n+1, n_rays), len (n_rays), end (n_rays, 4)."""
+               solout=SOLOUT_STATE, nthreads=0, trajectories=True, stage_order=STAGE_DIRECT,
+               time_order=TIME_ACCUMULATE, combo_order=COMBO_SCALE_SIXTH):
+    """Step-major SoA: dict with t (n+1), x/y/kx/ky (n+1, n_rays), len (n_rays), end (n_rays, 4).
+
+    stage_order / time_order select the rounding order of the RK4 stage
+    states (STAGE_DIRECT y + k*(dt/2), STAGE_SCALE_DT y + (k*dt)/2,
+    STAGE_HALF_FIRST y + (k*0.5)*dt), of the time column
+    (TIME_ACCUMULATE t += dt, TIME_PRODUCT t0 + i*dt) and of the step's
+    combination (COMBO_SCALE_SIXTH sum*(dt/6), COMBO_DT_THEN_SIX (sum*dt)/6)."""
     x0, y0, kx0, ky0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (x0, y0, kx0, ky0))
     nr = x0.size
     n = num_steps(t0, t_end, dt)
@@ -302,7 +320,8 @@ def trace_many(b, c, x0, y0, kx0, ky0, t0, t_end, dt, prec=PREC_REFERENCE,
     outs = [np.empty((n + 1, nr)) if trajectories else None for _ in range(4)]
     ln = np.empty(nr, dtype=np.int32)
     end = np.empty((nr, 4))
-    rc = lib().or_trace_many(ctypes.byref(b.s), ctypes.byref(c.s), prec, solout, nr,
+    rc = lib().or_trace_many_ordered(ctypes.byref(b.s), ctypes.byref(c.s), prec, solout,
+                                     stage_order, time_order, combo_order, nr,
                              _p(x0, _f64p), _p(y0, _f64p), _p(kx0, _f64p), _p(ky0, _f64p),
                              t0, t_end, dt, nthreads, _p(t, _f64p),
                              *[_p(o, _f64p) for o in outs], _p(ln, _i32p), _p(end, _f64p))
@@ -315,10 +334,12 @@ def trace_many(b, c, x0, y0, kx0, ky0, t0, t_end, dt, prec=PREC_REFERENCE,
 
 
 def fd_end_ld(b, c, s0, t0, t_end, dt, eps=0.0, ds0=None, ddepth=None, du=None, dv=None,
-              nthreads=0):
+              nthreads=0, with_cells=False):
     """Long-double traces from s0 +- eps ds0 with values +- eps d (fp64-interp
     algorithm): returns diff = end(+) - end(-), mid = (end(+) + end(-))/2
-    (n_rays, 4) and the two row counts."""
+    (n_rays, 4) and the two row counts; with_cells adds a boolean per ray,
+    True where both traces queried the same grid cells at every evaluation
+    (their central difference spans no cell line)."""
     s0 = np.ascontiguousarray(s0, dtype=np.float64).reshape(-1, 4)
     nr = s0.shape[0]
     arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
@@ -327,11 +348,15 @@ def fd_end_ld(b, c, s0, t0, t_end, dt, eps=0.0, ds0=None, ddepth=None, du=None, 
     mid = np.empty((nr, 4))
     lp = np.empty(nr, dtype=np.int32)
     lm = np.empty(nr, dtype=np.int32)
+    same = np.empty(nr, dtype=np.int32)
     rc = lib().or_fd_end_ld(ctypes.byref(b.s), ctypes.byref(c.s), nr, _p(s0, _f64p),
                             *[_p(a, _f64p) for a in arrs], eps, t0, t_end, dt, nthreads,
-                            _p(diff, _f64p), _p(mid, _f64p), _p(lp, _i32p), _p(lm, _i32p))
+                            _p(diff, _f64p), _p(mid, _f64p), _p(lp, _i32p), _p(lm, _i32p),
+                            _p(same, _i32p))
     if rc != 0:
         raise ValueError("invalid step count (the reference panics)")
+    if with_cells:
+        return diff, mid, lp, lm, same.astype(bool)
     return diff, mid, lp, lm
 
 

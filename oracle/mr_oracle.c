@@ -522,9 +522,30 @@ int64_t or_num_steps(double t0, double t_end, double dt)
     return (int64_t)q;
 }
 
+/* The stage state y + k h for h = dt/2 or dt, in one of the orders the
+ * unvendored ode_solvers 0.4.0 might use (which one it uses is parity
+ * unpinned): OR_STAGE_DIRECT y + k*(dt/2) (and y + k*dt), OR_STAGE_SCALE_DT
+ * y + (k*dt)/2, OR_STAGE_HALF_FIRST y + (k*0.5)*dt.  For h = dt the three
+ * are the same expression. */
+static double stage_state(int order, double y, double k, double dt, double half, int is_half)
+{
+    if (!is_half) return y + k * dt;
+    switch (order) {
+    case OR_STAGE_SCALE_DT: return y + (k * dt) / 2.0;
+    case OR_STAGE_HALF_FIRST: return y + (k * 0.5) * dt;
+    default: return y + k * half;
+    }
+}
+
 /* ray.rs:198-213 + ode_solvers Rk4 (new/integrate/step) + solout
- * (wave_ray_path.rs:236-246).  Returns rows written or -1 / -2 (panic). */
+ * (wave_ray_path.rs:236-246).  stage_order: see stage_state; time_order
+ * OR_TIME_ACCUMULATE t += dt (Rk4 keeps x = x + step_size) or
+ * OR_TIME_PRODUCT t0 + i*dt; combo_order OR_COMBO_SCALE_SIXTH
+ * y + (((k1 + k2*2) + k3*2) + k4)*(dt/6) or OR_COMBO_DT_THEN_SIX
+ * y + ((((k1 + k2*2) + k3*2) + k4)*dt)/6.  Returns rows written or -1 / -2
+ * (panic). */
 static int64_t trace_one(const or_bathy *b, const or_current *c, int prec, int solout,
+                         int stage_order, int time_order, int combo_order,
                          const double s0[4], double t0, double dt, int64_t n,
                          int64_t max_rows, double *t_rows, double *s_rows)
 {
@@ -541,15 +562,17 @@ static int64_t trace_one(const or_bathy *b, const or_current *c, int prec, int s
         if (have_next) { memcpy(k1, tmp, sizeof k1); st = OR_OK; }
         else st = system_st(b, c, prec, y, k1);
         if (st == OR_PANIC) return -2;
-        for (int q = 0; q < 4; q++) tmp[q] = y[q] + k1[q] * half;
+        for (int q = 0; q < 4; q++) tmp[q] = stage_state(stage_order, y[q], k1[q], dt, half, 1);
         if (system_st(b, c, prec, tmp, k2) == OR_PANIC) return -2;
-        for (int q = 0; q < 4; q++) tmp[q] = y[q] + k2[q] * half;
+        for (int q = 0; q < 4; q++) tmp[q] = stage_state(stage_order, y[q], k2[q], dt, half, 1);
         if (system_st(b, c, prec, tmp, k3) == OR_PANIC) return -2;
-        for (int q = 0; q < 4; q++) tmp[q] = y[q] + k3[q] * dt;
+        for (int q = 0; q < 4; q++) tmp[q] = stage_state(stage_order, y[q], k3[q], dt, half, 0);
         if (system_st(b, c, prec, tmp, k4) == OR_PANIC) return -2;
-        for (int q = 0; q < 4; q++)
-            yn[q] = y[q] + (((k1[q] + k2[q] * 2.0) + k3[q] * 2.0) + k4[q]) * sixth;
-        t = t + dt;
+        for (int q = 0; q < 4; q++) {
+            const double sum = ((k1[q] + k2[q] * 2.0) + k3[q] * 2.0) + k4[q];
+            yn[q] = y[q] + (combo_order == OR_COMBO_DT_THEN_SIX ? (sum * dt) / 6.0 : sum * sixth);
+        }
+        t = time_order == OR_TIME_PRODUCT ? t0 + (double)(i + 1) * dt : t + dt;
         if (rows < max_rows) { t_rows[rows] = t; memcpy(s_rows + 4 * rows, yn, sizeof yn); }
         rows++;
         memcpy(y, yn, sizeof y);
@@ -574,7 +597,8 @@ int64_t or_trace_single(const or_bathy *b, const or_current *c, int prec, int so
     double *tr = (double *)malloc(sizeof(double) * (size_t)cap);
     double *sr = (double *)malloc(sizeof(double) * 4 * (size_t)cap);
     if (!tr || !sr) { free(tr); free(sr); return -1; }
-    int64_t rows = trace_one(b, c, prec, solout, s0, t0, dt, n, cap, tr, sr);
+    int64_t rows = trace_one(b, c, prec, solout, OR_STAGE_DIRECT, OR_TIME_ACCUMULATE,
+                             OR_COMBO_SCALE_SIXTH, s0, t0, dt, n, cap, tr, sr);
     if (rows > 0) {
         for (int64_t r = 0; r < rows && r < max_rows; r++) {
             rows_out[5 * r] = tr[r];
@@ -586,13 +610,14 @@ int64_t or_trace_single(const or_bathy *b, const or_current *c, int prec, int so
 }
 
 /* ray.rs:98-127: one task per ray (rayon par_iter) -> OpenMP dynamic,1 */
-int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
-                  int64_t n_rays, const double *x0, const double *y0,
-                  const double *kx0, const double *ky0,
-                  double t0, double t_end, double dt, int nthreads,
-                  double *t_out, double *x_out, double *y_out,
-                  double *kx_out, double *ky_out, int32_t *len_out,
-                  double *end_out)
+int or_trace_many_ordered(const or_bathy *b, const or_current *c, int prec, int solout,
+                          int stage_order, int time_order, int combo_order,
+                          int64_t n_rays, const double *x0, const double *y0,
+                          const double *kx0, const double *ky0,
+                          double t0, double t_end, double dt, int nthreads,
+                          double *t_out, double *x_out, double *y_out,
+                          double *kx_out, double *ky_out, int32_t *len_out,
+                          double *end_out)
 {
     int64_t n = or_num_steps(t0, t_end, dt);
     if (n < 0) return -1;
@@ -600,7 +625,10 @@ int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
     if (t_out) {
         double t = t0;
         t_out[0] = t;
-        for (int64_t s = 1; s <= n; s++) { t = t + dt; t_out[s] = t; }
+        for (int64_t s = 1; s <= n; s++) {
+            t = time_order == OR_TIME_PRODUCT ? t0 + (double)s * dt : t + dt;
+            t_out[s] = t;
+        }
     }
     int panicked = 0;
 #ifdef _OPENMP
@@ -615,7 +643,8 @@ int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
 #endif
         for (int64_t r = 0; r < n_rays; r++) {
             double s0[4] = {x0[r], y0[r], kx0[r], ky0[r]};
-            int64_t rows = trace_one(b, c, prec, solout, s0, t0, dt, n, rows_max, tr, sr);
+            int64_t rows = trace_one(b, c, prec, solout, stage_order, time_order, combo_order,
+                                     s0, t0, dt, n, rows_max, tr, sr);
             if (rows < 0) { panicked = 1; rows = 0; }
             if (len_out) len_out[r] = (int32_t)rows;
             for (int64_t s = 0; s < rows_max; s++) {
@@ -634,6 +663,20 @@ int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
         free(tr); free(sr);
     }
     return panicked ? -2 : 0;
+}
+
+int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
+                  int64_t n_rays, const double *x0, const double *y0,
+                  const double *kx0, const double *ky0,
+                  double t0, double t_end, double dt, int nthreads,
+                  double *t_out, double *x_out, double *y_out,
+                  double *kx_out, double *ky_out, int32_t *len_out,
+                  double *end_out)
+{
+    return or_trace_many_ordered(b, c, prec, solout, OR_STAGE_DIRECT, OR_TIME_ACCUMULATE,
+                                 OR_COMBO_SCALE_SIXTH, n_rays,
+                                 x0, y0, kx0, ky0, t0, t_end, dt, nthreads, t_out, x_out, y_out,
+                                 kx_out, ky_out, len_out, end_out);
 }
 
 void or_system_many(const or_bathy *b, const or_current *c, int prec, int64_t n,

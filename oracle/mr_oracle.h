@@ -30,6 +30,12 @@ enum { OR_PREC_REFERENCE = 0, OR_PREC_FP64_INTERP = 1 };
 /* solout variants (which derivative ode_solvers hands to solout) */
 enum { OR_SOLOUT_STATE = 0, OR_SOLOUT_NEXT_DERIV = 1 };
 
+/* stage-state and time orders of the RK4 step (parity unpinned; the
+ * default is the first of each) */
+enum { OR_STAGE_DIRECT = 0, OR_STAGE_SCALE_DT = 1, OR_STAGE_HALF_FIRST = 2 };
+enum { OR_TIME_ACCUMULATE = 0, OR_TIME_PRODUCT = 1 };
+enum { OR_COMBO_SCALE_SIXTH = 0, OR_COMBO_DT_THEN_SIX = 1 };
+
 /* bathymetry kinds */
 enum { OR_BATHY_CONSTANT = 0, OR_BATHY_SLOPE = 1, OR_BATHY_ARRAY = 2, OR_BATHY_GRID = 3 };
 /* current kinds */
@@ -98,6 +104,18 @@ int or_trace_many(const or_bathy *b, const or_current *c, int prec, int solout,
                   double *kx_out, double *ky_out, int32_t *len_out,
                   double *end_out);
 
+/* or_trace_many with another stage-state order (y + k*(dt/2), y + (k*dt)/2
+ * or y + (k*0.5)*dt), time column (t += dt or t0 + i*dt) and step
+ * combination (sum*(dt/6) or (sum*dt)/6). */
+int or_trace_many_ordered(const or_bathy *b, const or_current *c, int prec, int solout,
+                          int stage_order, int time_order, int combo_order,
+                          int64_t n_rays, const double *x0, const double *y0,
+                          const double *kx0, const double *ky0,
+                          double t0, double t_end, double dt, int nthreads,
+                          double *t_out, double *x_out, double *y_out,
+                          double *kx_out, double *ky_out, int32_t *len_out,
+                          double *end_out);
+
 /* Batched RHS evaluation for golden-vector checks: ds_out[4*i..] = system(s). */
 void or_system_many(const or_bathy *b, const or_current *c, int prec, int64_t n,
                     const double *s, double *ds_out, int32_t *status_out);
@@ -109,12 +127,15 @@ int or_max_threads(void);
  * s0 + eps ds0 and from s0 - eps ds0 (s0, ds0: 4 per ray; ds0 may be NULL)
  * with the gridded values + eps d and - eps d (ddepth, du, dv may be NULL);
  * diff = end(+) - end(-) and mid = (end(+) + end(-))/2, each formed in long
- * double, 4 per ray; len_plus / len_minus the row counts (may be NULL).
+ * double, 4 per ray; len_plus / len_minus the row counts (may be NULL);
+ * same_cells (may be NULL) 1 where the two traces queried the same grid
+ * cells at every evaluation, 0 where one crossed a cell line the other did
+ * not (the central difference then spans a kink of the cell slopes).
  * Returns 0, or -1 on a bad step count. */
 int or_fd_end_ld(const or_bathy *b, const or_current *c, int64_t n_rays, const double *s0,
                  const double *ds0, const double *ddepth, const double *du, const double *dv,
                  double eps, double t0, double t_end, double dt, int nthreads, double *diff,
-                 double *mid, int32_t *len_plus, int32_t *len_minus);
+                 double *mid, int32_t *len_plus, int32_t *len_minus, int32_t *same_cells);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,22 @@ typedef struct {
     const ld *depth, *u, *v;      /* gridded values (perturbed), or NULL */
 } fields_ld;
 
+/* The signature of the cells a trace's queries use, in order (FNV-1a over
+ * each query's cell indices).  Two traces with the same signature met the
+ * same cells at every evaluation, so the cell-difference slopes and the
+ * bilinear pieces they used are the same smooth functions: a central
+ * difference between them differentiates through no kink. */
+static void sig_mix(uint64_t *sig, int64_t v)
+{
+    if (!sig) return;
+    uint64_t h = *sig;
+    for (int q = 0; q < 8; q++) {
+        h ^= (uint64_t)(v >> (8 * q)) & 0xFFu;
+        h *= 0x100000001b3ull;
+    }
+    *sig = h;
+}
+
 /* cartesian_current.rs:231-253 in long double on an axis of n points */
 static int nearest_ld(ld target, ld a0, ld a1, int64_t n, ld *idx)
 {
@@ -90,13 +106,15 @@ static int bilinear_ld(const ld p[4][3], ld tx, ld ty, ld *out)
  * arrays and their cell-difference slopes */
 static int grid_ld(int64_t nx, int64_t ny, ld x0a, ld x1a, ld y0a, ld y1a,
                    const ld *ax, const ld *ay, int64_t len_a, const ld *va, int64_t len_b,
-                   const ld *vb, ld x, ld y, ld out[6])
+                   const ld *vb, ld x, ld y, ld out[6], uint64_t *sig)
 {
     ld xi, yi;
     int64_t i1, j1;
     if (nearest_ld(x, x0a, x1a, nx, &xi) != OR_OK) return OR_ERR;
     if (nearest_ld(y, y0a, y1a, ny, &yi) != OR_OK) return OR_ERR;
     if (cell_ld(xi, nx, &i1) != OR_OK || cell_ld(yi, ny, &j1) != OR_OK) return OR_ERR;
+    sig_mix(sig, i1);
+    sig_mix(sig, j1);
     const int64_t ci[4][2] = {{i1, j1}, {i1, j1 + 1}, {i1 + 1, j1 + 1}, {i1 + 1, j1}};
     ld pa[4][3], pb[4][3];
     for (int q = 0; q < 4; q++) {
@@ -121,7 +139,8 @@ typedef struct {
     ld *bx, *by, *cx, *cy;        /* axes widened */
 } axes_ld;
 
-static int bathy_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld *h, ld *hx, ld *hy)
+static int bathy_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld *h, ld *hx, ld *hy,
+                    uint64_t *sig)
 {
     const or_bathy *b = F->b;
     switch (b->kind) {
@@ -143,13 +162,14 @@ static int bathy_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld *h, ld 
     if (isnan(x) || isnan(y)) { *h = NAN; *hx = NAN; *hy = NAN; return OR_OK; }
     ld out[6];
     if (grid_ld(b->nx, b->ny, A->bx[0], A->bx[1], A->by[0], A->by[1], A->bx, A->by,
-                b->depth_len, F->depth, 0, NULL, x, y, out) != OR_OK)
+                b->depth_len, F->depth, 0, NULL, x, y, out, sig) != OR_OK)
         return OR_ERR;
     *h = out[0]; *hx = out[1]; *hy = out[2];
     return OR_OK;
 }
 
-static int current_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld cv[6])
+static int current_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld cv[6],
+                      uint64_t *sig)
 {
     const or_current *c = F->c;
     if (c->kind == OR_CURRENT_CONSTANT) {
@@ -158,17 +178,18 @@ static int current_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld cv[6]
         return OR_OK;
     }
     return grid_ld(c->nx, c->ny, A->cx[0], A->cx[1], A->cy[0], A->cy[1], A->cx, A->cy,
-                   c->u_len, F->u, c->v_len, F->v, x, y, cv);
+                   c->u_len, F->u, c->v_len, F->v, x, y, cv, sig);
 }
 
 /* wave_ray_path.rs:118-150 with group_velocity :177-188 and dkdt_bathy
  * :207-216; any error -> all-NaN (:222-227) */
-static void system_ld(const fields_ld *F, const axes_ld *A, const ld s[4], ld ds[4])
+static void system_ld(const fields_ld *F, const axes_ld *A, const ld s[4], ld ds[4],
+                      uint64_t *sig)
 {
     ld h, hx, hy, cv[6];
     ds[0] = ds[1] = ds[2] = ds[3] = NAN;
-    if (bathy_ld(F, A, s[0], s[1], &h, &hx, &hy) != OR_OK) return;
-    if (current_ld(F, A, s[0], s[1], cv) != OR_OK) return;
+    if (bathy_ld(F, A, s[0], s[1], &h, &hx, &hy, sig) != OR_OK) return;
+    if (current_ld(F, A, s[0], s[1], cv, sig) != OR_OK) return;
     ld kx = s[2], ky = s[3];
     ld k = sqrtl(kx * kx + ky * ky);
     ld theta = atan2l(ky, kx);
@@ -191,20 +212,23 @@ static int all_nan4_ld(const ld v[4])
     return isnan(v[0]) && isnan(v[1]) && isnan(v[2]) && isnan(v[3]);
 }
 
-/* ode_solvers Rk4 with the MR_SOLOUT_STATE stop; returns the row count */
-static int64_t trace_ld(const fields_ld *F, const axes_ld *A, ld y[4], ld dt, int64_t n)
+/* ode_solvers Rk4 with the MR_SOLOUT_STATE stop; returns the row count and
+ * leaves the cell signature of every query in *sig */
+static int64_t trace_ld(const fields_ld *F, const axes_ld *A, ld y[4], ld dt, int64_t n,
+                        uint64_t *sig)
 {
     const ld half = dt / 2.0L, sixth = dt / 6.0L;
     ld k1[4], k2[4], k3[4], k4[4], tmp[4];
     int64_t rows = 1;
+    *sig = 0xcbf29ce484222325ull;
     for (int64_t i = 0; i < n; i++) {
-        system_ld(F, A, y, k1);
+        system_ld(F, A, y, k1, sig);
         for (int q = 0; q < 4; q++) tmp[q] = y[q] + k1[q] * half;
-        system_ld(F, A, tmp, k2);
+        system_ld(F, A, tmp, k2, sig);
         for (int q = 0; q < 4; q++) tmp[q] = y[q] + k2[q] * half;
-        system_ld(F, A, tmp, k3);
+        system_ld(F, A, tmp, k3, sig);
         for (int q = 0; q < 4; q++) tmp[q] = y[q] + k3[q] * dt;
-        system_ld(F, A, tmp, k4);
+        system_ld(F, A, tmp, k4, sig);
         for (int q = 0; q < 4; q++)
             y[q] = y[q] + (((k1[q] + k2[q] * 2.0L) + k3[q] * 2.0L) + k4[q]) * sixth;
         rows++;
@@ -239,7 +263,7 @@ static ld *widen_f64(const double *v, int64_t n)
 int or_fd_end_ld(const or_bathy *b, const or_current *c, int64_t n_rays, const double *s0,
                  const double *ds0, const double *ddepth, const double *du, const double *dv,
                  double eps, double t0, double t_end, double dt, int nthreads, double *diff,
-                 double *mid, int32_t *len_plus, int32_t *len_minus)
+                 double *mid, int32_t *len_plus, int32_t *len_minus, int32_t *same_cells)
 {
     int64_t n = or_num_steps(t0, t_end, dt);
     if (n < 0) return -1;
@@ -270,8 +294,10 @@ int or_fd_end_ld(const or_bathy *b, const or_current *c, int64_t n_rays, const d
             yp[q] = (ld)s0[4 * r + q] + d;
             ym[q] = (ld)s0[4 * r + q] - d;
         }
-        int64_t lp = trace_ld(&Fp, &A, yp, (ld)dt, n);
-        int64_t lm = trace_ld(&Fm, &A, ym, (ld)dt, n);
+        uint64_t sp, sm;
+        int64_t lp = trace_ld(&Fp, &A, yp, (ld)dt, n, &sp);
+        int64_t lm = trace_ld(&Fm, &A, ym, (ld)dt, n, &sm);
+        if (same_cells) same_cells[r] = sp == sm && lp == lm;
         for (int q = 0; q < 4; q++) {
             diff[4 * r + q] = (double)(yp[q] - ym[q]);
             mid[4 * r + q] = (double)((yp[q] + ym[q]) * 0.5L);
