@@ -245,6 +245,34 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current,
                     double *g_x0, double *g_y0, double *g_kx0, double *g_ky0,
                     double *g_depth, double *g_u, double *g_v);
 
+/* Size of the device array mr_adjoint_many_noalloc uses as its work buffer:
+ * 32 MiB per device, held from the library's first use of it on a device
+ * until the process ends.  A launch fits when mr_adjoint_workspace_size
+ * (n_rays, stride) is at most this: e.g. up to 2,621 rays at stride 50, or
+ * 13,107 rays at stride 10. */
+#define MR_ADJOINT_SCRATCH_BYTES (32ll << 20)
+
+/* mr_adjoint_many without a work argument, for callers that cannot
+ * allocate device memory (code under HIP-graph capture, serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  The work
+ * buffer is a static device array of MR_ADJOINT_SCRATCH_BYTES; sizes that
+ * need more are refused with MR_ERR_INVALID_ARGUMENT.  The call allocates
+ * nothing: it does not build field tables either, so the fields must already
+ * hold their fp64 tables on this device (any mr_trace_fp64 call on the
+ * device builds them; the checkpoints this call needs come from one).  Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (other streams, other threads) race and give wrong gradients.
+ * Arguments, outputs and results are mr_adjoint_many's. */
+int mr_adjoint_many_noalloc(const mr_field *bathy, const mr_field *current,
+                            const double *depth, const double *u, const double *v,
+                            int64_t n_rays, double t0, double t_end, double dt, int32_t stride,
+                            const mr_options *opt, const double *ckpt, const int32_t *len,
+                            const double *end, const double *g_end,
+                            const double *g_x, const double *g_y, const double *g_kx,
+                            const double *g_ky,
+                            double *g_x0, double *g_y0, double *g_kx0, double *g_ky0,
+                            double *g_depth, double *g_u, double *g_v);
+
 /* ---- batched scenes: one persistent launch ----------------------------
  *
  * No counterpart in the reference, which traces one scene per call

@@ -28,6 +28,7 @@ MR_PREC_REFERENCE, MR_PREC_FP64_INTERP = 0, 1
 MR_MATH_FAST, MR_MATH_LITERAL = 0, 1
 MR_SOLOUT_STATE, MR_SOLOUT_NEXT_DERIV = 0, 1
 MR_MEM_HOST, MR_MEM_DEVICE = 0, 1
+MR_ADJOINT_SCRATCH_BYTES = 32 << 20
 
 # every symbol include/mantaray_capi.h declares
 EXPORTS = (
@@ -37,7 +38,8 @@ EXPORTS = (
     "mr_current_constant", "mr_field_free", "mr_field_kind", "mr_field_grid_shape",
     "mr_field_copy_grid", "mr_field_upload", "mr_num_steps", "mr_trace_many",
     "mr_single_ray", "mr_rhs_many", "mr_checkpoint_count", "mr_adjoint_workspace_size",
-    "mr_trace_fp64", "mr_adjoint_many", "mr_ray_result_json", "mr_ray_result_append_tsv",
+    "mr_trace_fp64", "mr_adjoint_many", "mr_adjoint_many_noalloc", "mr_ray_result_json",
+    "mr_ray_result_append_tsv",
     "mr_batch_create", "mr_batch_run", "mr_batch_info", "mr_batch_free",
 )
 
@@ -122,6 +124,10 @@ _SIGS = {
                                        ctypes.c_int32, ctypes.POINTER(Options), _vp, _vp, _vp,
                                        _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp]),
+    "mr_adjoint_many_noalloc": (ctypes.c_int, [_fieldp, _fieldp, _vp, _vp, _vp, _i64, _f64, _f64,
+                                               _f64, ctypes.c_int32, ctypes.POINTER(Options),
+                                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, _vp, _vp, _vp, _vp]),
     "mr_ray_result_json": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp, _i64,
                                           _i64p]),
     "mr_ray_result_append_tsv": (ctypes.c_int, [_cs, _vp, _vp, _vp, _vp, _vp, _i64, _i64]),

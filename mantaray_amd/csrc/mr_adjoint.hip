@@ -21,10 +21,46 @@
 // with the forward's own step code (bitwise the forward's values) into a
 // per-lane column of the work buffer, then runs the step adjoints in reverse
 // order over them.
+#include <mutex>
+
+#include "mantaray_capi.h"
 #include "mr_kernels.h"
 #include "mr_kernel_common.h"
 
 namespace mr {
+
+// The work buffer of mr_adjoint_many_noalloc: a file-scope device array of
+// MR_ADJOINT_SCRATCH_BYTES that each device holds while the library is
+// loaded.  Host code never uses the array's name (in host code it is a
+// stand-in, not a device address): adjoint_scratch looks its address up on
+// each device once, with hipGetSymbolAddress, and every later use goes
+// through that pointer.
+static __device__ __align__(256) unsigned char scratch[MR_ADJOINT_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t adjoint_scratch(void **ptr)
+{
+    *ptr = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 // One RK4 step in fp64 mode from s: the first three stage derivatives k and
 // the new state o.  The arithmetic is that of trace_kernel's step, and so is

@@ -549,6 +549,25 @@ int field_tables64(const mr_field *cf, const DevTables **out)
     return MR_OK;
 }
 
+// the fp64 tables of a field on the current device if they exist; builds
+// nothing (for the entry point that must not allocate)
+int existing_tables64(const mr_field *cf, const DevTables **out)
+{
+    mr_field *f = const_cast<mr_field *>(cf);
+    int dev = 0;
+    MR_HIP_TRY(hipGetDevice(&dev));
+    const bool gridded = f->kind == MR_BATHY_GRID || f->kind == MR_CURRENT_GRID;
+    const int key = dev * 16 + (gridded ? kDefaultTile : 0);
+    std::lock_guard<std::mutex> lk(f->mu);
+    auto it = f->dev.find(key);
+    if (it == f->dev.end() || !it->second->view64)
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "a field has no fp64 tables on this device: trace it with mr_trace_fp64 "
+                    "on the device first (this entry point builds nothing)");
+    *out = it->second.get();
+    return MR_OK;
+}
+
 int check_pair(const mr_field *b, const mr_field *c)
 {
     if (!b || !c) return fail(MR_ERR_INVALID_ARGUMENT, "null field handle");
@@ -739,6 +758,107 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
                             stream));
+    return MR_OK;
+}
+
+// mr_adjoint_many and mr_adjoint_many_noalloc: with no_alloc the work buffer
+// is this device's scratch array and no field table is built
+int adjoint_run(const mr_field *bathy, const mr_field *current, const double *depth,
+                const double *u, const double *v, int64_t n_rays, double t0, double t_end,
+                double dt, int32_t stride, const mr_options *opt, const double *ckpt,
+                const int32_t *len, const double *end, const double *g_end, const double *g_x,
+                const double *g_y, const double *g_kx, const double *g_ky, void *work,
+                int64_t work_bytes, bool no_alloc, double *g_x0, double *g_y0, double *g_kx0,
+                double *g_ky0, double *g_depth, double *g_u, double *g_v)
+{
+    int rc = check_pair(bathy, current);
+    if (rc) return rc;
+    mr_options o;
+    if ((rc = resolve_options(opt, o))) return rc;
+    if (o.memory != MR_MEM_DEVICE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "mr_adjoint_many takes device memory (MR_MEM_DEVICE)");
+    if (o.solout != MR_SOLOUT_STATE)
+        return fail(MR_ERR_INVALID_ARGUMENT, "the adjoint supports MR_SOLOUT_STATE only");
+    if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
+    if (g_depth && bathy->kind != MR_BATHY_GRID)
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "depth gradient requested of a non-gridded bathymetry");
+    if ((g_u || g_v) && current->kind != MR_CURRENT_GRID)
+        return fail(MR_ERR_INVALID_ARGUMENT, "current gradient requested of a non-gridded current");
+    int64_t need = 0;
+    mr_adjoint_workspace_size(n_rays, stride, &need);
+    if (no_alloc && need > MR_ADJOINT_SCRATCH_BYTES)
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "the sweep needs " + std::to_string(need) + " bytes of work buffer (" +
+                        std::to_string(n_rays) + " rays at stride " + std::to_string(stride) +
+                        "); mr_adjoint_many_noalloc holds " +
+                        std::to_string((long long)MR_ADJOINT_SCRATCH_BYTES));
+    if (n_rays > 0 && (!ckpt || !len || !end || (!no_alloc && !work)))
+        return fail(MR_ERR_INVALID_ARGUMENT, "null checkpoint, length, end or work buffer");
+    if (!no_alloc && work_bytes < need)
+        return fail(MR_ERR_INVALID_ARGUMENT, "work buffer too small");
+    int64_t n;
+    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
+    DeviceGuard g(o.device);
+    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
+    const DevTables *bt, *ct;
+    if (no_alloc) {
+        if ((rc = existing_tables64(bathy, &bt)) || (rc = existing_tables64(current, &ct)))
+            return rc;
+        MR_HIP_TRY(adjoint_scratch(&work));
+    } else if ((rc = field_tables64(bathy, &bt)) || (rc = field_tables64(current, &ct))) {
+        return rc;
+    }
+    // the sweep kernel by workgroups (64 rays each) per CU: four waves per
+    // workgroup up to one per CU (latency-bound: recompute | two linearise |
+    // chain), two up to two per CU (recompute | sweep), one beyond (each wave
+    // recomputes and sweeps in turn, the least SIMD time once the launch fills
+    // the GPU); block_size 64, 128 or 256 forces one, two or four
+    int cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    const int64_t groups = (n_rays + 63) / 64;
+    int waves = groups <= cus ? 4 : groups <= 2 * (int64_t)cus ? 2 : 1;
+    if (o.block_size == 64 || o.block_size == 128 || o.block_size == 256)
+        waves = o.block_size / 64;
+    else if (o.block_size != 0)
+        return fail(MR_ERR_INVALID_ARGUMENT,
+                    "mr_adjoint_many: block_size must be 0, 64, 128 or 256");
+    hipStream_t stream = (hipStream_t)o.stream;
+    if (g_depth) MR_HIP_TRY(launch_fill(g_depth, (int64_t)bathy->a.size(), 0.0, stream));
+    if (g_u) MR_HIP_TRY(launch_fill(g_u, (int64_t)current->a.size(), 0.0, stream));
+    if (g_v) MR_HIP_TRY(launch_fill(g_v, (int64_t)current->b.size(), 0.0, stream));
+    AdjointArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.B = bt->view64;
+    a.C = ct->view64;
+    a.bp = bt->p64;
+    a.cp = ct->p64;
+    a.f.depth = depth ? depth : bt->va;
+    a.f.u = u ? u : ct->va;
+    a.f.v = v ? v : ct->vb;
+    a.n_rays = n_rays;
+    a.n_steps = n;
+    a.dt = dt;
+    a.half = dt / 2.0;
+    a.sixth = dt / 6.0;
+    a.stride = stride;
+    a.ckpt = ckpt;
+    a.len = len;
+    a.end = end;
+    a.g_end = g_end;
+    a.g_x = g_x; a.g_y = g_y; a.g_kx = g_kx; a.g_ky = g_ky;
+    a.work = static_cast<double *>(work);
+    a.work_ld = n_rays;
+    a.g_x0 = g_x0; a.g_y0 = g_y0; a.g_kx0 = g_kx0; a.g_ky0 = g_ky0;
+    a.grads.depth = g_depth;
+    a.grads.u = g_u;
+    a.grads.v = g_v;
+    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, waves,
+                              stream));
     return MR_OK;
 }
 
@@ -1292,82 +1412,25 @@ int mr_adjoint_many(const mr_field *bathy, const mr_field *current, const double
                     double *g_ky0, double *g_depth, double *g_u, double *g_v)
 {
     MR_GUARD_BEGIN
-    int rc = check_pair(bathy, current);
-    if (rc) return rc;
-    mr_options o;
-    if ((rc = resolve_options(opt, o))) return rc;
-    if (o.memory != MR_MEM_DEVICE)
-        return fail(MR_ERR_INVALID_ARGUMENT, "mr_adjoint_many takes device memory (MR_MEM_DEVICE)");
-    if (o.solout != MR_SOLOUT_STATE)
-        return fail(MR_ERR_INVALID_ARGUMENT, "the adjoint supports MR_SOLOUT_STATE only");
-    if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
-    if (g_depth && bathy->kind != MR_BATHY_GRID)
-        return fail(MR_ERR_INVALID_ARGUMENT,
-                    "depth gradient requested of a non-gridded bathymetry");
-    if ((g_u || g_v) && current->kind != MR_CURRENT_GRID)
-        return fail(MR_ERR_INVALID_ARGUMENT, "current gradient requested of a non-gridded current");
-    int64_t need = 0;
-    mr_adjoint_workspace_size(n_rays, stride, &need);
-    if (n_rays > 0 && (!ckpt || !len || !end || !work))
-        return fail(MR_ERR_INVALID_ARGUMENT, "null checkpoint, length, end or work buffer");
-    if (work_bytes < need) return fail(MR_ERR_INVALID_ARGUMENT, "work buffer too small");
-    int64_t n;
-    if ((rc = mr_num_steps(t0, t_end, dt, &n))) return rc;
-    DeviceGuard g(o.device);
-    if (!g.ok) return fail(MR_ERR_DEVICE, "cannot select device");
-    const DevTables *bt, *ct;
-    if ((rc = field_tables64(bathy, &bt)) || (rc = field_tables64(current, &ct))) return rc;
-    hipStream_t stream = (hipStream_t)o.stream;
-    if (g_depth) MR_HIP_TRY(launch_fill(g_depth, (int64_t)bathy->a.size(), 0.0, stream));
-    if (g_u) MR_HIP_TRY(launch_fill(g_u, (int64_t)current->a.size(), 0.0, stream));
-    if (g_v) MR_HIP_TRY(launch_fill(g_v, (int64_t)current->b.size(), 0.0, stream));
-    AdjointArgs a;
-    std::memset(&a, 0, sizeof a);
-    a.B = bt->view64;
-    a.C = ct->view64;
-    a.bp = bt->p64;
-    a.cp = ct->p64;
-    a.f.depth = depth ? depth : bt->va;
-    a.f.u = u ? u : ct->va;
-    a.f.v = v ? v : ct->vb;
-    a.n_rays = n_rays;
-    a.n_steps = n;
-    a.dt = dt;
-    a.half = dt / 2.0;
-    a.sixth = dt / 6.0;
-    a.stride = stride;
-    a.ckpt = ckpt;
-    a.len = len;
-    a.end = end;
-    a.g_end = g_end;
-    a.g_x = g_x; a.g_y = g_y; a.g_kx = g_kx; a.g_ky = g_ky;
-    a.work = static_cast<double *>(work);
-    a.work_ld = n_rays;
-    a.g_x0 = g_x0; a.g_y0 = g_y0; a.g_kx0 = g_kx0; a.g_ky0 = g_ky0;
-    a.grads.depth = g_depth;
-    a.grads.u = g_u;
-    a.grads.v = g_v;
-    // the sweep kernel by workgroups (64 rays each) per CU: four waves per
-    // workgroup up to one per CU (latency-bound: recompute | two linearise |
-    // chain), two up to two per CU (recompute | sweep), one beyond (each wave
-    // recomputes and sweeps in turn, the least SIMD time once the launch fills
-    // the GPU); block_size 64, 128 or 256 forces one, two or four
-    int cus = 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-        cus = 256;
-    const int64_t groups = (n_rays + 63) / 64;
-    int waves = groups <= cus ? 4 : groups <= 2 * (int64_t)cus ? 2 : 1;
-    if (o.block_size == 64 || o.block_size == 128 || o.block_size == 256)
-        waves = o.block_size / 64;
-    else if (o.block_size != 0)
-        return fail(MR_ERR_INVALID_ARGUMENT,
-                    "mr_adjoint_many: block_size must be 0, 64, 128 or 256");
-    MR_HIP_TRY(launch_adjoint(a, bathy_kind_of(bathy), current_kind_of(current), o.math, waves,
-                              stream));
-    return MR_OK;
+    return adjoint_run(bathy, current, depth, u, v, n_rays, t0, t_end, dt, stride, opt, ckpt, len,
+                       end, g_end, g_x, g_y, g_kx, g_ky, work, work_bytes, false, g_x0, g_y0,
+                       g_kx0, g_ky0, g_depth, g_u, g_v);
+    MR_GUARD_END
+}
+
+int mr_adjoint_many_noalloc(const mr_field *bathy, const mr_field *current, const double *depth,
+                            const double *u, const double *v, int64_t n_rays, double t0,
+                            double t_end, double dt, int32_t stride, const mr_options *opt,
+                            const double *ckpt, const int32_t *len, const double *end,
+                            const double *g_end, const double *g_x, const double *g_y,
+                            const double *g_kx, const double *g_ky, double *g_x0, double *g_y0,
+                            double *g_kx0, double *g_ky0, double *g_depth, double *g_u,
+                            double *g_v)
+{
+    MR_GUARD_BEGIN
+    return adjoint_run(bathy, current, depth, u, v, n_rays, t0, t_end, dt, stride, opt, ckpt, len,
+                       end, g_end, g_x, g_y, g_kx, g_ky, nullptr, 0, true, g_x0, g_y0, g_kx0,
+                       g_ky0, g_depth, g_u, g_v);
     MR_GUARD_END
 }
 

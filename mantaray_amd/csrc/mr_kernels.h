@@ -150,5 +150,8 @@ hipError_t launch_adjoint(const AdjointArgs &a, int bk, int ck, int math, int wa
 hipError_t launch_rhs64(const Rhs64Args &a, int bk, int ck, int math, hipStream_t stream);
 hipError_t launch_build_m64(const M64BuildArgs &a, hipStream_t stream);
 hipError_t launch_fill(double *p, int64_t n, double v, hipStream_t stream);
+// the device address of this device's MR_ADJOINT_SCRATCH_BYTES scratch array
+// (looked up on the device's first call, then cached)
+hipError_t adjoint_scratch(void **ptr);
 
 }  // namespace mr
