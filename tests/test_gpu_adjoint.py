@@ -488,3 +488,79 @@ def test_sweep_kernels_agree(mr, torch, case):
             d = np.abs(a2 - a4).max() / max(np.abs(a2).max(), 1e-300)
             print(f"{case} {k}: two vs {block // 64} waves {d:.2e}")
             assert d <= 1e-12, (k, block, d)
+
+
+def test_adjoint_eddy_per_ray(mr, oracle_mod, torch):
+    """The eddy workload's gradients ray by ray.  Four reverse sweeps over
+    all 16,384 rays (the four-wave kernel this launch size selects), each
+    with a unit cotangent on one end-state component of the 256 subset rays
+    and zero elsewhere, give each subset ray's 4 x 4 Jacobian of its end
+    state with respect to its initial state; the same Jacobian comes from
+    central differences of the long-double trace (eps 1e-6 m for positions,
+    1e-12 for wavenumbers).  Per ray and end-state component, the row of
+    the Jacobian (columns scaled by those eps, so that positions and
+    wavenumbers weigh alike) must agree within 1e-6 of its largest entry.
+    Rays whose two perturbed traces met different grid cells (a jump of the
+    cell slopes lies between them) are left out and counted; on this
+    workload there are none.  The loss gradient of each ray, normwise over
+    its four components scaled alike, must agree within 1e-6 of the size of
+    its terms 2 dx . x' + 2 dy . y' (which can cancel): a component that is a
+    small entry of its Jacobian row carries that row's error, so a relative
+    test component by component would measure the row's conditioning."""
+    from mantaray_amd import adjoint, workloads
+    cfg = workloads.eddy_adjoint()
+    b, c = workloads.make_fields(cfg)
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    ic = [_dev(torch, cfg[k]) for k in ("x0", "y0", "kx0", "ky0")]
+    u = _dev(torch, cfg["current"]["u"].ravel())
+    v = _dev(torch, cfg["current"]["v"].ravel())
+    T = cfg["t_end"]
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, 1.0, stride=50, u=u, v=v)
+    nr = ic[0].numel()
+    sub = np.arange(256)
+    J = np.zeros((256, 4, 4))                     # J[r, q, j] = d end_q / d s0_j
+    for q in range(4):
+        ge = torch.zeros((nr, 4), dtype=torch.float64, device="cuda:0")
+        ge[:256, q] = 1.0
+        g = adjoint.adjoint(b, c, fwd, 0.0, T, 1.0, 50, g_end=ge, u=u, v=v,
+                            want=("x0", "y0", "kx0", "ky0"))
+        for j, k in enumerate(("x0", "y0", "kx0", "ky0")):
+            J[:, q, j] = g[k][:256].cpu().numpy()
+    torch.cuda.synchronize()
+    s0 = np.stack([cfg[k][sub] for k in ("x0", "y0", "kx0", "ky0")], 1)
+    scale = np.array([1e-6, 1e-6, 1e-12, 1e-12])
+    Jfd = np.zeros_like(J)
+    same = np.ones(256, dtype=bool)
+    mids = None
+    for j in range(4):
+        ds0 = np.zeros_like(s0)
+        ds0[:, j] = 1.0
+        diff, mid, lp, lm, sc = oracle_mod.fd_end_ld(ob, oc, s0, 0.0, T, 1.0, eps=scale[j],
+                                                     ds0=ds0, with_cells=True)
+        Jfd[:, :, j] = diff / (2 * scale[j])
+        same &= sc & (lp == lm)
+        mids = mid
+    print(f"eddy per ray: {int((~same).sum())} of 256 rays crossed a cell line between "
+          f"their perturbed traces")
+    assert same.sum() >= 250
+    A, B = J * scale, Jfd * scale                 # columns in units of the perturbations
+    err = np.abs(A - B).max(axis=2) / np.abs(B).max(axis=2)     # per ray and end component
+    for q, name in enumerate(("x", "y", "kx", "ky")):
+        e = err[same, q]
+        print(f"eddy per-ray Jacobian row d{name}(T)/ds0: max {e.max():.2e} "
+              f"p99 {np.quantile(e, 0.99):.2e} median {np.median(e):.2e}")
+    assert err[same].max() <= 1e-6, float(err[same].max())
+    # the loss gradient per ray: normwise over the ray's four components
+    # (scaled as the Jacobian's columns), against the size of its terms
+    # 2 dx . x' and 2 dy . y' (which can cancel)
+    tx, ty = cfg["target_x"][sub], cfg["target_y"][sub]
+    dx, dy = 2 * (mids[:, 0] - tx), 2 * (mids[:, 1] - ty)
+    g_loss = (dx[:, None] * J[:, 0, :] + dy[:, None] * J[:, 1, :]) * scale
+    f_loss = (dx[:, None] * Jfd[:, 0, :] + dy[:, None] * Jfd[:, 1, :]) * scale
+    terms = (np.abs(dx[:, None] * Jfd[:, 0, :]) + np.abs(dy[:, None] * Jfd[:, 1, :])) * scale
+    rel = np.abs(g_loss - f_loss).max(axis=1) / terms.max(axis=1)
+    comp = np.abs(g_loss - f_loss) / terms
+    print(f"eddy per-ray loss gradient: max normwise error / terms {rel[same].max():.2e} "
+          f"(p99 {np.quantile(rel[same], 0.99):.2e}); one component at a time "
+          f"max {comp[same].max():.2e}")
+    assert rel[same].max() <= 1e-6
