@@ -6,6 +6,7 @@
 #include "mr_guard.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -53,6 +54,7 @@ struct DevTables {
     Grid64Ptrs p64{};
     const double *va = nullptr;    // the field's own values: depth, or u
     const double *vb = nullptr;    // v
+    const SCell *scells = nullptr; // combined records with a current on shared cells
     ~DevTables()
     {
         for (void *p : allocs) (void)hipFree(p);
@@ -112,6 +114,12 @@ int64_t first_oob_cell(int64_t nx, int64_t ny, int64_t len)
     return ncx * (ny - 1);
 }
 
+uint64_t next_field_uid()
+{
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+}
+
 }  // namespace
 
 namespace mr {
@@ -120,6 +128,7 @@ int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace mr
 
 struct mr_field {
+    uint64_t uid = next_field_uid();   // distinct for every field the process creates
     int32_t kind = 0;
     float h = 0, h0 = 0, x0 = 0, y0 = 0, dhdx = 0, dhdy = 0;
     double u = 0, v = 0;
@@ -131,6 +140,9 @@ struct mr_field {
     std::vector<double> a, b;       // depth, or u and v
     std::mutex mu;
     std::map<int, std::unique_ptr<DevTables>> dev;
+    // a bathymetry's combined records with a current on shared cells, per
+    // (device * 16 + record tile, current uid)
+    std::map<std::pair<int, uint64_t>, std::unique_ptr<DevTables>> shared;
 };
 
 struct mr_batch {
@@ -720,6 +732,103 @@ bool shared_cells(const mr_field *b, const mr_field *c)
     return true;
 }
 
+// The fast region of each cell along one axis (SCell): the f32 targets p in
+// [lo, hi] lie in cell i for the depth's f32 index rule (T32[i] <= p <
+// T32[i+1]), for every f64 coordinate that rounds to p lie in cell i for the
+// current's f64 index rule (T64[i] <= x < T64[i+1]), and are strictly
+// between the cell's node coordinates a[i] and a[i+1] (no corner).  If
+// RN32(x) >= lo > ceil32(T64[i]) then x > T64[i]; if RN32(x) <= hi <
+// floor32(T64[i+1]) then x < T64[i+1] (rounding is monotone).
+std::vector<float2> fast_bounds(const std::vector<float> &a32, const std::vector<double> &a64)
+{
+    const std::vector<float> t32 = thresholds32(a32);
+    const std::vector<double> t64 = thresholds64(a64);
+    auto up = [](float f) { return std::nextafter(f, INFINITY); };
+    auto down = [](float f) { return std::nextafter(f, -INFINITY); };
+    auto ceil32 = [&](double d) {
+        float f = (float)d;
+        return (double)f < d ? up(f) : f;
+    };
+    auto floor32 = [&](double d) {
+        float f = (float)d;
+        return (double)f > d ? down(f) : f;
+    };
+    std::vector<float2> b(a32.size() - 1);
+    for (size_t i = 0; i + 1 < a32.size(); i++) {
+        const float lo = std::max({t32[i], up(a32[i]), up(ceil32(t64[i]))});
+        const float hi = std::min({down(t32[i + 1]), down(a32[i + 1]), down(floor32(t64[i + 1]))});
+        b[i] = make_float2(lo, hi);
+    }
+    return b;
+}
+
+// the combined records of a depth and a current on shared cells (SCell) on
+// the current device, built on first use and kept with the bathymetry
+int shared_tables(const mr_field *cb, const mr_field *cc, int tl, const SCell **out)
+{
+    const DevTables *bt, *ct;
+    int rc = field_tables(cb, tl, &bt);
+    if (!rc) rc = field_tables(cc, tl, &ct);
+    if (rc) return rc;
+    mr_field *b = const_cast<mr_field *>(cb);
+    int dev = 0;
+    MR_HIP_TRY(hipGetDevice(&dev));
+    const std::pair<int, uint64_t> key(dev * 16 + tl, cc->uid);
+    std::lock_guard<std::mutex> lk(b->mu);
+    auto it = b->shared.find(key);
+    if (it != b->shared.end()) {
+        *out = it->second->scells;
+        return MR_OK;
+    }
+    std::unique_ptr<DevTables> t(new DevTables);
+    std::vector<double> cx(cc->xd), cy(cc->yd);
+    std::vector<float2> fx = fast_bounds(b->xf, cx), fy = fast_bounds(b->yf, cy);
+    float2 *dfx, *dfy;
+    if ((rc = dev_upload(*t, fx.data(), fx.size(), &dfx)) ||
+        (rc = dev_upload(*t, fy.data(), fy.size(), &dfy)))
+        return rc;
+    const int64_t ntx = tiles_along(b->nx, tl), nty = tiles_along(b->ny, tl);
+    SCell *cells;
+    if ((rc = validate_grid(b->nx, b->ny, (int64_t)sizeof(SCell), tl)) ||
+        (rc = dev_alloc(*t, (size_t)((ntx * nty) << (2 * tl)), &cells)))
+        return rc;
+    SharedBuildArgs a;
+    a.nx = b->nx; a.ny = b->ny;
+    // the depth's axis pairs and raw values, and the current's raw values,
+    // are already on the device with the fields' own tables
+    std::vector<float2> xp = axis_pairs(b->xf), yp = axis_pairs(b->yf);
+    float2 *dxp, *dyp;
+    if ((rc = dev_upload(*t, xp.data(), xp.size(), &dxp)) ||
+        (rc = dev_upload(*t, yp.data(), yp.size(), &dyp)))
+        return rc;
+    a.xp = dxp; a.yp = dyp; a.fx = dfx; a.fy = dfy;
+    a.depth = bt->bp.raw; a.u = ct->cp.raw_u; a.v = ct->cp.raw_v;
+    a.oob_from = std::min({first_oob_cell(b->nx, b->ny, (int64_t)b->a.size()),
+                           first_oob_cell(b->nx, b->ny, (int64_t)cc->a.size()),
+                           first_oob_cell(b->nx, b->ny, (int64_t)cc->b.size())});
+    a.bxspace = (double)b->xf[1] - (double)b->xf[0];      // cartesian_netcdf3.rs:119-120
+    a.byspace = (double)b->yf[1] - (double)b->yf[0];
+    a.cxspace = cc->xd[1] - cc->xd[0];                      // cartesian_current.rs:512-513
+    a.cyspace = cc->yd[1] - cc->yd[0];
+    a.ntx = (int)(ntx << 4 | tl);
+    a.cells = cells;
+    MR_HIP_TRY(launch_build_shared(a, 0));
+    MR_HIP_TRY(hipDeviceSynchronize());
+    t->scells = cells;
+    *out = cells;
+    b->shared[key] = std::move(t);
+    return MR_OK;
+}
+
+// shared cells whose combined records the kernels can index with 32-bit
+// byte offsets (grids up to about 5790 x 5790 cells)
+bool use_shared(const mr_field *b, const mr_field *c, int tl)
+{
+    return shared_cells(b, c) &&
+           (tiles_along(b->nx, tl) * tiles_along(b->ny, tl) << (2 * tl)) * (int64_t)sizeof(SCell) <
+               (int64_t(1) << 32);
+}
+
 // the trace itself, all pointers device pointers, enqueued on stream
 int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
                  const double *x0, const double *y0, const double *kx0, const double *ky0,
@@ -753,7 +862,10 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     a.len_out = len_out;
     a.end_out = end_out;
     a.solout = o.solout;
-    a.shared_cells = shared_cells(bathy, current) ? 1 : 0;
+    a.shared_cells = use_shared(bathy, current, o.record_tile) ? 1 : 0;
+    a.scells = nullptr;
+    if (a.shared_cells && (rc = shared_tables(bathy, current, o.record_tile, &a.scells)))
+        return rc;
     const int block = o.block_size ? o.block_size : tuned_block(n_rays);
     a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
@@ -1261,7 +1373,7 @@ int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *
         if (i == 0) { b->bk = bk; b->ck = ck; }
         else if (bk != b->bk || ck != b->ck)
             return fail(MR_ERR_INVALID_ARGUMENT, "scenes of one batch must share field kinds");
-        b->shared_cells = b->shared_cells && shared_cells(sc.bathy, sc.current);
+        b->shared_cells = b->shared_cells && use_shared(sc.bathy, sc.current, o.record_tile);
         if (sc.n_rays < 0) return fail(MR_ERR_INVALID_ARGUMENT, "negative ray count");
         if (sc.n_rays > 0 && (!sc.x0 || !sc.y0 || !sc.kx0 || !sc.ky0 || !sc.len_out || !sc.end_out))
             return fail(MR_ERR_INVALID_ARGUMENT, "null scene arrays");
@@ -1278,11 +1390,17 @@ int mr_batch_create(int32_t n_scenes, const mr_scene *scenes, const mr_options *
         d.ray0 = b->n_rays;
         d.x0 = sc.x0; d.y0 = sc.y0; d.kx0 = sc.kx0; d.ky0 = sc.ky0;
         d.len_out = sc.len_out; d.end_out = sc.end_out;
+        d.scells = nullptr;
         b->n_rays += sc.n_rays;
         b->max_ray_steps += sc.n_rays * n;
         ray_scene.insert(ray_scene.end(), (size_t)sc.n_rays, i);
     }
     if (b->n_rays >= (int64_t(1) << 31)) return fail(MR_ERR_UNSUPPORTED, "too many rays");
+    if (b->shared_cells)
+        for (int32_t i = 0; i < n_scenes; i++)
+            if ((rc = shared_tables(scenes[i].bathy, scenes[i].current, o.record_tile,
+                                    &desc[(size_t)i].scells)))
+                return rc;
     if ((rc = dev_upload(*b->mem, desc.data(), desc.size(), &b->scenes)) ||
         (rc = dev_upload(*b->mem, ray_scene.data(), ray_scene.size(), &b->ray_scene)) ||
         (rc = dev_alloc(*b->mem, 1, &b->next)))

@@ -52,6 +52,29 @@ struct CCell {                    // 64 B
     double ux, uy, vx, vy;        // cartesian_current.rs:515-536
 };
 
+// One cell of a gridded depth and a gridded current that share their cells
+// (same node counts; the current's axes rounded to f32 are the depth's), in
+// one 128-byte record, so that one load and one test serve both fields.
+// [lo_x, hi_x] x [lo_y, hi_y] is the cell's fast region: a target whose f32
+// rounding (px, py) lies in it is in this cell for the depth's f32 index
+// rule, and, for every f64 point that rounds to (px, py), for the current's
+// f64 index rule; and it is on no corner (px, py are strictly between the
+// cell's node coordinates).  Such a target needs no other test: the depth
+// and both current components are the cell's bilinear pieces at (px, py),
+// with the cell's slopes.  Cells where the reference errs (a corner past a
+// value array, det == 0) get an empty region.  Each bound is found on the
+// host from the exact threshold tables of both index rules.
+struct SCell {                    // 128 B
+    float lo_x, hi_x, lo_y, hi_y; // fast region (f32, closed)
+    float x0, y0, m01, m10;       // cell origin (f32 node coordinates); change of basis
+    float a00, a10, a01, a11;     // depth, interpolator.rs:78-81 over (f32)depth
+    float hx, hy;                 // depth slopes (f32), cartesian_netcdf3.rs:119-134
+    int i1, j1;                   // the cell
+    float ua00, ua10, ua01, ua11; // u over (f32)u
+    float va00, va10, va01, va11; // v over (f32)v
+    double ux, uy, vx, vy;        // current slopes (f64), cartesian_current.rs:515-536
+};
+
 struct BathyView {
     int kind;
     float h;                              // ConstantDepth
@@ -598,15 +621,6 @@ __device__ __forceinline__ double bathy_fast(const BathySlot &bs, double x, doub
     return __builtin_isnan(r1.z) ? MR_NAN : (double)hv;    // det == 0 errs
 }
 
-//
-// SC (the depth and current grids share their cells, i.e. the current's
-// axes rounded to f32 are the depth's): the current's corner test is left
-// out, because whenever both cell tests pass it equals the depth's.  The two
-// slots then hold the same cell or, for a target within rounding distance of
-// a node line, the two cells beside that line; their corner sets along that
-// axis share the node on it, and the other corners lie a cell away from the
-// target.  Along the other axis the same holds.
-template <bool SC = false>
 __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, double y, double &u,
                                              double &v, double &ux, double &uy, double &vxg,
                                              double &vyg, bool &slow)
@@ -619,16 +633,94 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
     const float4 r0 = cs.r0, r1 = cs.r1;
     const float uf = bilin_eval(r0.x, r0.y, r0.z, r0.w, uw, ww);
     const float vf = bilin_eval(r1.x, r1.y, r1.z, r1.w, uw, ww);
-    if (SC) {
-        slow = !ok;
-    } else {
-        const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
-        slow = !ok | on_corner(dxa, dxd, dya, dyb);
-    }
+    const float dxd = tx - cs.xp.y, dyb = ty - cs.yp.y;
+    slow = !ok | on_corner(dxa, dxd, dya, dyb);
     u = (double)uf;
     v = (double)vf;
     ux = cs.g0.x; uy = cs.g0.y; vxg = cs.g1.x; vyg = cs.g1.y;
     return !__builtin_isnan(cs.m.x);                       // det == 0 errs
+}
+
+// ---------------------------------------------------------------------------
+// Shared-cell records (SCell) in registers: eight 16-byte pieces, held as
+// f64 pairs so that copies between slots move 64 bits per instruction.
+
+struct SharedSlot {
+    mr_d2 q[8] = {{MR_NAN_INIT, MR_NAN_INIT}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0},
+                  {0.0, 0.0},                 {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+};
+
+// the four f32 values of a 16-byte record piece.  The whole piece is bit-cast
+// at once: hipcc 7.2 (clang 20) miscompiles a bit cast of one element of a
+// by-value double vector, d.y, to a float pair, returning d.x's halves.
+__device__ __forceinline__ mr_f4 f4of(mr_d2 d)
+{
+    return __builtin_bit_cast(mr_f4, d);
+}
+
+__device__ __forceinline__ void shared_load(const SCell *cells, SharedSlot &s, uint32_t cell)
+{
+    const MR_GLOBAL mr_d2 *p = (const MR_GLOBAL mr_d2 *)((const MR_GLOBAL char *)cells + cell * 128u);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s.q[k] = p[k];
+}
+
+// (px, py) in the slot's fast region; a never-loaded slot has NaN bounds
+// and contains no point, and so does a NaN target
+__device__ __forceinline__ bool shared_in_region(const SharedSlot &s, float px, float py)
+{
+    const mr_f4 b = f4of(s.q[0]);              // (lo_x, hi_x, lo_y, hi_y)
+    return (px >= b.x) & (px <= b.y) & (py >= b.z) & (py <= b.w);
+}
+
+// Both fields at (x, y) from the slot's record, for a target in its fast
+// region: the operations bathy_query and current_query take there
+// (interpolator.rs:39-84 with u = m01 tty, w = m10 ttx; the u and v pieces
+// at the same (u, w)), and slow set where the target is outside the region.
+__device__ __forceinline__ void shared_fast(const SharedSlot &s, double x, double y, double &h,
+                                            double &hx, double &hy, double &u, double &v,
+                                            double &ux, double &uy, double &vx, double &vy,
+                                            bool &slow)
+{
+    const float px = (float)x, py = (float)y;
+    slow = !shared_in_region(s, px, py);
+    const mr_f4 om = f4of(s.q[1]);             // (x0, y0, m01, m10)
+    const float tx = px - om.x, ty = py - om.y;
+    const float uu = om.z * ty, ww = om.w * tx;
+    const mr_f4 d = f4of(s.q[2]), g = f4of(s.q[3]), uc = f4of(s.q[4]), vc = f4of(s.q[5]);
+    const float hv = bilin_eval(d.x, d.y, d.z, d.w, uu, ww);
+    const float uf = bilin_eval(uc.x, uc.y, uc.z, uc.w, uu, ww);
+    const float vf = bilin_eval(vc.x, vc.y, vc.z, vc.w, uu, ww);
+    h = (double)hv;
+    hx = (double)g.x;
+    hy = (double)g.y;
+    u = (double)uf;
+    v = (double)vf;
+    ux = s.q[6].x; uy = s.q[6].y; vx = s.q[7].x; vy = s.q[7].y;
+}
+
+// WaveRayPath::system for gridded depth and current on shared cells: every
+// lane evaluates from the record in hand; lanes outside its fast region
+// evaluate again through the full queries (bathy_query, current_query),
+// which find and load their cells' records and handle nodes and errors.
+template <int MATH>
+__device__ __forceinline__ void rhs_shared(const BathyView &B, const BathyPtrs &BP,
+                                           const CurrentView &C, const CurrentPtrs &CP,
+                                           const SharedSlot &ss, const FastConsts &K, double x,
+                                           double y, double kx, double ky, double &dx,
+                                           double &dy, double &dkx, double &dky)
+{
+    double h, hx, hy, u, v, ux, uy, vx, vy;
+    bool slow;
+    shared_fast(ss, x, y, h, hx, hy, u, v, ux, uy, vx, vy, slow);
+    odes<MATH>(K, h, hx, hy, true, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+    if (slow) {
+        BathySlot bs;
+        CurrentSlot cs;
+        const double h2 = bathy_query<BK_GRID>(B, BP, bs, x, y, hx, hy);
+        const bool okc = current_query<CK_GRID>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
+        odes<MATH>(K, h2, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+    }
 }
 
 // WaveRayPath::system (wave_ray_path.rs:220-234) around odes (:118-150).
@@ -637,7 +729,7 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
 // interpolations with the dispersion arithmetic); the lanes it cannot
 // answer then evaluate again with the full queries, which reload records
 // and read corner values.
-template <int BK, int CK, int MATH, bool SC = false>
+template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, BathySlot &bs,
                                     const CurrentView &C, const CurrentPtrs &CP,
                                     CurrentSlot &cs, const FastConsts &K, double x, double y,
@@ -649,7 +741,7 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
         bool sb = false, sc = false;
         const double h = BK == BK_GRID ? bathy_fast(bs, x, y, hx, hy, sb)
                                        : bathy_query<BK>(B, BP, bs, x, y, hx, hy);
-        const bool okc = CK == CK_GRID ? current_fast<SC>(cs, x, y, u, v, ux, uy, vx, vy, sc)
+        const bool okc = CK == CK_GRID ? current_fast(cs, x, y, u, v, ux, uy, vx, vy, sc)
                                        : current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
         odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
         if (sb | sc) {

@@ -23,6 +23,7 @@ struct TraceArgs {
     double *end_out;                   // 4 per ray, may be null
     int solout;                        // 0 state, 1 next derivative
     int shared_cells;                  // gridded depth and current on one set of cells
+    const SCell *scells;               // their combined records (with shared_cells)
 };
 
 // One scene of a batched trace, in device memory.
@@ -36,6 +37,7 @@ struct SceneDesc {
     const double *x0, *y0, *kx0, *ky0; // the scene's initial conditions
     int32_t *len_out;                  // the scene's outputs
     double *end_out;
+    const SCell *scells;               // combined records (batches with shared_cells)
 };
 
 // A persistent batched trace: rays of every scene taken from one queue.
@@ -66,6 +68,19 @@ struct BathyBuildArgs {
     double xspace, yspace;
     int ntx;                           // (record tiles along x) << 4 | log2 tile edge
     BCell *cells;
+};
+
+// combined records of a gridded depth and current on shared cells
+struct SharedBuildArgs {
+    int64_t nx, ny;
+    const float2 *xp, *yp;             // the depth's f32 axis pairs (a[i], a[i+1])
+    const float2 *fx, *fy;             // per axis cell: the fast region's (lo, hi)
+    const double *depth, *u, *v;       // raw values
+    int64_t oob_from;                  // cells from here on have a corner past some array
+    double bxspace, byspace;           // (f64)x[1] - (f64)x[0] of the f32 axes (depth slopes)
+    double cxspace, cyspace;           // x[1] - x[0] of the f64 axes (current slopes)
+    int ntx;                           // as in BathyBuildArgs
+    SCell *cells;
 };
 
 struct CurrentBuildArgs {
@@ -141,6 +156,7 @@ hipError_t launch_batch(const BatchArgs &a, int bk, int ck, int math, int block,
 hipError_t launch_clear_counter(unsigned long long *p, hipStream_t stream);
 hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream);
 hipError_t launch_build_current(const CurrentBuildArgs &a, hipStream_t stream);
+hipError_t launch_build_shared(const SharedBuildArgs &a, hipStream_t stream);
 hipError_t launch_trace64(const Trace64Args &a, int bk, int ck, int math, int block,
                           hipStream_t stream);
 // waves per 64 rays: 2 (recompute | sweep) or 4 (recompute | two linearise |

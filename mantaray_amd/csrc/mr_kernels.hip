@@ -33,12 +33,8 @@ namespace mr {
 // stop_before when (MR_SOLOUT_NEXT_DERIV) the first stage derivative is
 // all-NaN, i.e. the ray already ended on the row it starts from.  bn, cn:
 // the records prefetched for the next evaluation; (px, py): the latest
-// dx/dt, dy/dt, for the guess.  SC: the depth and current grids have the
-// same cells (same node counts, current axes rounding to the depth axes), so
-// the depth record's cell test and cell guess serve both prefetches (a
-// prefetch is only a guess: the queries check the cell they use), and the
-// depth's corner test serves both fields (current_fast).  No result changes.
-template <int BK, int CK, int MATH, bool SC = false>
+// dx/dt, dy/dt, for the guess.
+template <int BK, int CK, int MATH>
 __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP, BathySlot &bn,
                                          const CurrentView &C, const CurrentPtrs &CP,
                                          CurrentSlot &cn, const FastConsts &K, double &px,
@@ -65,22 +61,13 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             cq = cn;
         } else {
             const double gx = x + px * hs, gy = y + py * hs;
-            if (SC) {
-                if (!bathy_in_cell(bq, (float)gx, (float)gy)) {
-                    const int i1 = guess_cell((float)gx, B.ax0, B.pinv_sx, B.pmax_x);
-                    const int j1 = guess_cell((float)gy, B.ay0, B.pinv_sy, B.pmax_y);
-                    bathy_load(B, BP, bn, i1, j1);
-                    current_load(C, CP, cn, i1, j1);
-                }
-            } else {
-                if (PF_B && !bathy_in_cell(bq, (float)gx, (float)gy))
-                    bathy_prefetch(B, BP, bn, gx, gy);
-                if (PF_C && !current_in_cell(cq, gx, gy))
-                    current_prefetch(C, CP, cn, gx, gy);
-            }
+            if (PF_B && !bathy_in_cell(bq, (float)gx, (float)gy))
+                bathy_prefetch(B, BP, bn, gx, gy);
+            if (PF_C && !current_in_cell(cq, gx, gy))
+                current_prefetch(C, CP, cn, gx, gy);
         }
         double dx, dy, dkx, dky;
-        rhs<BK, CK, MATH, SC>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        rhs<BK, CK, MATH>(B, BP, bq, C, CP, cq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
             // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
@@ -100,6 +87,66 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
     }
     o[0] = x + ax * sixth; o[1] = y + ay * sixth;
     o[2] = kx + akx * sixth; o[3] = ky + aky * sixth;
+}
+
+// rk4_step for gridded depth and current on shared cells: one combined
+// record (SCell) per cell serves both fields.  sn: the record prefetched for
+// the next evaluation, loaded at stages 1 and 3 when the guessed point of the
+// pair that follows lies outside the fast region of the record in hand (a
+// prefetch is only a guess: rhs_shared tests the region it uses, and lanes
+// outside it take the full queries).
+template <int MATH>
+__device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyPtrs &BP,
+                                                const CurrentView &C, const CurrentPtrs &CP,
+                                                const SCell *cells, SharedSlot &sn,
+                                                const FastConsts &K, double &px, double &py,
+                                                double dt, double half, double sixth, int solout,
+                                                double x, double y, double kx, double ky,
+                                                double o[4], bool &stop_before)
+{
+    double sx = x, sy = y, skx = kx, sky = ky;   // stage state
+    double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
+    stop_before = false;
+    SharedSlot sq = sn;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const double hs = st < 2 ? half : dt;
+        if (st == 1 || st == 3) {
+            sq = sn;
+        } else {
+            const float gx = (float)(x + px * hs), gy = (float)(y + py * hs);
+            if (!shared_in_region(sq, gx, gy)) {
+                const int i1 = guess_cell(gx, B.ax0, B.pinv_sx, B.pmax_x);
+                const int j1 = guess_cell(gy, B.ay0, B.pinv_sy, B.pmax_y);
+                shared_load(cells, sn, tiled_cell(i1, j1, B.ntx));
+            }
+        }
+        double dx, dy, dkx, dky;
+        rhs_shared<MATH>(B, BP, C, CP, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        px = dx; py = dy;
+        if (st == 0) {
+            stop_before = solout == 1 && all_nan4(dx, dy, dkx, dky);
+            ax = dx; ay = dy; akx = dkx; aky = dky;
+        } else if (st < 3) {
+            ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
+            akx = __builtin_fma(dkx, 2.0, akx); aky = __builtin_fma(dky, 2.0, aky);
+        } else {
+            ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
+        }
+        sx = x + dx * hs; sy = y + dy * hs;
+        skx = kx + dkx * hs; sky = ky + dky * hs;
+    }
+    o[0] = x + ax * sixth; o[1] = y + ay * sixth;
+    o[2] = kx + akx * sixth; o[3] = ky + aky * sixth;
+}
+
+// the first record of a lane: the cell its start most likely lies in
+__device__ __forceinline__ void shared_prefetch(const BathyView &B, const SCell *cells,
+                                                SharedSlot &s, double x, double y)
+{
+    const int i1 = guess_cell((float)x, B.ax0, B.pinv_sx, B.pmax_x);
+    const int j1 = guess_cell((float)y, B.ay0, B.pinv_sy, B.pmax_y);
+    shared_load(cells, s, tiled_cell(i1, j1, B.ntx));
 }
 
 // Lanes past n_rays repeat the last ray (same inputs, same arithmetic, so
@@ -140,10 +187,16 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     settle(K);
     BathySlot bn;                      // records prefetched for the next evaluation
     CurrentSlot cn;
-    if (PF_B) bathy_prefetch(B, a.bp, bn, x, y);
-    if (PF_C) current_prefetch(C, a.cp, cn, x, y);
-    if (PF_B) settle(bn);              // likewise for the first records
-    if (PF_C) settle(cn);
+    SharedSlot sn;                     // the same, for shared cells
+    if (SC) {
+        shared_prefetch(B, a.scells, sn, x, y);
+        settle(sn);                    // likewise for the first record
+    } else {
+        if (PF_B) bathy_prefetch(B, a.bp, bn, x, y);
+        if (PF_C) current_prefetch(C, a.cp, cn, x, y);
+        if (PF_B) settle(bn);
+        if (PF_C) settle(cn);
+    }
     double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
     bool alive = true;
     int64_t len = 1;
@@ -154,8 +207,12 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (alive) {
             double o[4];
             bool stop_before;
-            rk4_step<BK, CK, MATH, SC>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
-                                   a.solout, x, y, kx, ky, o, stop_before);
+            if constexpr (SC)
+                rk4_step_shared<MATH>(B, a.bp, C, a.cp, a.scells, sn, K, px, py, dt, half, sixth,
+                                      a.solout, x, y, kx, ky, o, stop_before);
+            else
+                rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
+                                       a.solout, x, y, kx, ky, o, stop_before);
             if (stop_before) {
                 alive = false;
             } else {
@@ -220,6 +277,8 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
     CurrentPtrs cp{};
     BathySlot bn;
     CurrentSlot cn;
+    SharedSlot sn;
+    const SCell *scells = nullptr;
     double x = 0.0, y = 0.0, kx = 0.0, ky = 0.0, px = 0.0, py = 0.0;
     double dt = 0.0, half = 0.0, sixth = 0.0;
     int64_t local = 0, n_steps = 0, step = 0, len = 1;
@@ -252,10 +311,15 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                     end_out = S->end_out;
                     x = S->x0[local]; y = S->y0[local]; kx = S->kx0[local]; ky = S->ky0[local];
                     step = 0; len = 1; px = 0.0; py = 0.0;
-                    bn = BathySlot{};
-                    cn = CurrentSlot{};
-                    if (PF_B) bathy_prefetch(B, bp, bn, x, y);
-                    if (PF_C) current_prefetch(C, cp, cn, x, y);
+                    if (SC) {
+                        scells = S->scells;
+                        shared_prefetch(B, scells, sn, x, y);
+                    } else {
+                        bn = BathySlot{};
+                        cn = CurrentSlot{};
+                        if (PF_B) bathy_prefetch(B, bp, bn, x, y);
+                        if (PF_C) current_prefetch(C, cp, cn, x, y);
+                    }
                     have = true;
                 } else {
                     done = true;
@@ -268,8 +332,12 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
             if (!finished) {
                 double o[4];
                 bool stop_before;
-                rk4_step<BK, CK, MATH, SC>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
-                                       a.solout, x, y, kx, ky, o, stop_before);
+                if constexpr (SC)
+                    rk4_step_shared<MATH>(B, bp, C, cp, scells, sn, K, px, py, dt, half, sixth,
+                                          a.solout, x, y, kx, ky, o, stop_before);
+                else
+                    rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
+                                           a.solout, x, y, kx, ky, o, stop_before);
                 ++step;
                 if (stop_before) {
                     finished = true;
@@ -391,6 +459,57 @@ __global__ void build_current_kernel(CurrentBuildArgs a)
     a.mtab[t] = cell_basis(a.xp, a.yp, i, j);
 }
 
+// Combined records of a depth and a current on shared cells (SCell): the
+// depth's coefficients and slopes as build_bathy_kernel makes them, the
+// current's as build_current_kernel does, the change of basis of the shared
+// f32 axes, and the cell's fast region from the per-axis bounds; empty for a
+// cell where either field errs (a corner past its data, det == 0).
+__global__ void build_shared_kernel(SharedBuildArgs a)
+{
+    const int64_t ncx = a.nx - 1;
+    const int64_t ncell = ncx * (a.ny - 1);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int64_t i = c % ncx, j = c / ncx;
+    SCell r;
+    const float2 m = cell_basis(a.xp, a.yp, i, j);   // m01 = NaN marks det == 0
+    const float2 xq = a.xp[i], yq = a.yp[j];
+    r.x0 = xq.x; r.y0 = yq.x; r.m01 = m.x; r.m10 = m.y;
+    r.i1 = (int)i; r.j1 = (int)j;
+    const bool err = c >= a.oob_from || __builtin_isnan(m.x);
+    if (c >= a.oob_from) {
+        r.a00 = r.a10 = r.a01 = r.a11 = r.hx = r.hy = MR_NANF;
+        r.ua00 = r.ua10 = r.ua01 = r.ua11 = MR_NANF;
+        r.va00 = r.va10 = r.va01 = r.va11 = MR_NANF;
+        r.ux = r.uy = r.vx = r.vy = MR_NAN;
+    } else {
+        const int64_t iA = a.nx * j + i, iB = iA + a.nx, iC = iA + a.nx + 1, iD = iA + 1;
+        float vA = (float)a.depth[iA], vB = (float)a.depth[iB];
+        float vC = (float)a.depth[iC], vD = (float)a.depth[iD];
+        r.a00 = vA;                                           // interpolator.rs:78-81
+        r.a10 = vB - vA;
+        r.a01 = vD - vA;
+        r.a11 = ((vC - vA) - r.a10) - r.a01;
+        r.hx = (float)((a.depth[iD] - a.depth[iA]) / a.bxspace);   // cartesian_netcdf3.rs:119-134
+        r.hy = (float)((a.depth[iB] - a.depth[iA]) / a.byspace);
+        vA = (float)a.u[iA]; vB = (float)a.u[iB]; vC = (float)a.u[iC]; vD = (float)a.u[iD];
+        r.ua00 = vA; r.ua10 = vB - vA; r.ua01 = vD - vA; r.ua11 = ((vC - vA) - r.ua10) - r.ua01;
+        vA = (float)a.v[iA]; vB = (float)a.v[iB]; vC = (float)a.v[iC]; vD = (float)a.v[iD];
+        r.va00 = vA; r.va10 = vB - vA; r.va01 = vD - vA; r.va11 = ((vC - vA) - r.va10) - r.va01;
+        r.ux = (a.u[iD] - a.u[iA]) / a.cxspace;               // cartesian_current.rs:515-536
+        r.uy = (a.u[iB] - a.u[iA]) / a.cyspace;
+        r.vx = (a.v[iD] - a.v[iA]) / a.cxspace;
+        r.vy = (a.v[iB] - a.v[iA]) / a.cyspace;
+    }
+    const float2 bx = a.fx[i], by = a.fy[j];
+    const float inf = __builtin_inff();
+    r.lo_x = err ? inf : bx.x;
+    r.hi_x = err ? -inf : bx.y;
+    r.lo_y = err ? inf : by.x;
+    r.hi_y = err ? -inf : by.y;
+    a.cells[tiled_cell((int)i, (int)j, a.ntx)] = r;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -486,6 +605,16 @@ hipError_t launch_build_bathy(const BathyBuildArgs &a, hipStream_t stream)
     if (ncell <= 0) return hipSuccess;
     const int block = 256;
     hipLaunchKernelGGL(build_bathy_kernel, dim3((unsigned)((ncell + block - 1) / block)),
+                       dim3(block), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_shared(const SharedBuildArgs &a, hipStream_t stream)
+{
+    const int64_t ncell = (a.nx - 1) * (a.ny - 1);
+    if (ncell <= 0) return hipSuccess;
+    const int block = 256;
+    hipLaunchKernelGGL(build_shared_kernel, dim3((unsigned)((ncell + block - 1) / block)),
                        dim3(block), 0, stream, a);
     return hipGetLastError();
 }
