@@ -68,8 +68,8 @@ struct SCell {                    // 128 B
     float lo_x, hi_x, lo_y, hi_y; // fast region (f32, closed)
     float x0, y0, m01, m10;       // cell origin (f32 node coordinates); change of basis
     float a00, a10, a01, a11;     // depth, interpolator.rs:78-81 over (f32)depth
-    float hx, hy;                 // depth slopes (f32), cartesian_netcdf3.rs:119-134
-    int i1, j1;                   // the cell
+    double hx, hy;                // depth slopes (f32 values, cartesian_netcdf3.rs:119-134,
+                                  // widened as the right-hand side widens them)
     float ua00, ua10, ua01, ua11; // u over (f32)u
     float va00, va10, va01, va11; // v over (f32)v
     double ux, uy, vx, vy;        // current slopes (f64), cartesian_current.rs:515-536
@@ -687,13 +687,13 @@ __device__ __forceinline__ void shared_fast(const SharedSlot &s, double x, doubl
     const mr_f4 om = f4of(s.q[1]);             // (x0, y0, m01, m10)
     const float tx = px - om.x, ty = py - om.y;
     const float uu = om.z * ty, ww = om.w * tx;
-    const mr_f4 d = f4of(s.q[2]), g = f4of(s.q[3]), uc = f4of(s.q[4]), vc = f4of(s.q[5]);
+    const mr_f4 d = f4of(s.q[2]), uc = f4of(s.q[4]), vc = f4of(s.q[5]);
     const float hv = bilin_eval(d.x, d.y, d.z, d.w, uu, ww);
     const float uf = bilin_eval(uc.x, uc.y, uc.z, uc.w, uu, ww);
     const float vf = bilin_eval(vc.x, vc.y, vc.z, vc.w, uu, ww);
     h = (double)hv;
-    hx = (double)g.x;
-    hy = (double)g.y;
+    hx = s.q[3].x;
+    hy = s.q[3].y;
     u = (double)uf;
     v = (double)vf;
     ux = s.q[6].x; uy = s.q[6].y; vx = s.q[7].x; vy = s.q[7].y;

@@ -11,6 +11,8 @@
 //                  (one lane: the accumulation is a sequential rounding chain),
 //                  for a trace with no rays; otherwise the trace kernel's last
 //                  workgroup writes it beside the rays.
+#include <cmath>
+
 #include "mr_kernels.h"
 #include "mr_kernel_common.h"
 
@@ -95,7 +97,11 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
 // pair that follows lies outside the fast region of the record in hand (a
 // prefetch is only a guess: rhs_shared tests the region it uses, and lanes
 // outside it take the full queries).
-template <int MATH>
+//
+// DY: dt and dt/2 are powers of two, so k * hs is exact (for every k whose
+// product is not subnormal) and the stage state y + k * hs of Rk4::step is
+// one rounding: fma(k, hs, y) gives it bitwise in one instruction.
+template <int MATH, bool DY>
 __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyPtrs &BP,
                                                 const CurrentView &C, const CurrentPtrs &CP,
                                                 const SCell *cells, SharedSlot &sn,
@@ -114,7 +120,10 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
         if (st == 1 || st == 3) {
             sq = sn;
         } else {
-            const float gx = (float)(x + px * hs), gy = (float)(y + py * hs);
+            // the guessed point in f32: it only chooses which record to load
+            const float fh = (float)hs;
+            const float gx = __builtin_fmaf((float)px, fh, (float)x);
+            const float gy = __builtin_fmaf((float)py, fh, (float)y);
             if (!shared_in_region(sq, gx, gy)) {
                 const int i1 = guess_cell(gx, B.ax0, B.pinv_sx, B.pmax_x);
                 const int j1 = guess_cell(gy, B.ay0, B.pinv_sy, B.pmax_y);
@@ -133,8 +142,13 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
         } else {
             ax = ax + dx; ay = ay + dy; akx = akx + dkx; aky = aky + dky;
         }
-        sx = x + dx * hs; sy = y + dy * hs;
-        skx = kx + dkx * hs; sky = ky + dky * hs;
+        if (DY) {
+            sx = __builtin_fma(dx, hs, x); sy = __builtin_fma(dy, hs, y);
+            skx = __builtin_fma(dkx, hs, kx); sky = __builtin_fma(dky, hs, ky);
+        } else {
+            sx = x + dx * hs; sy = y + dy * hs;
+            skx = kx + dkx * hs; sky = ky + dky * hs;
+        }
     }
     o[0] = x + ax * sixth; o[1] = y + ay * sixth;
     o[2] = kx + akx * sixth; o[3] = ky + aky * sixth;
@@ -153,7 +167,10 @@ __device__ __forceinline__ void shared_prefetch(const BathyView &B, const SCell 
 // their stores write the same bytes to the same addresses): every lane
 // stores every row, with no branch around the stores.  OUT: 0 no
 // trajectories, 1 all four components, 2 any subset.
-template <int BK, int CK, int MATH, int OUT, bool SC = false>
+// SC: 0 separate records per field; 1 the shared-cell records (SCell) of a
+// depth and current on the same cells; 2 the same with dt a power of two
+// (rk4_step_shared's DY).
+template <int BK, int CK, int MATH, int OUT, int SC = 0>
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
@@ -207,9 +224,10 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (alive) {
             double o[4];
             bool stop_before;
-            if constexpr (SC)
-                rk4_step_shared<MATH>(B, a.bp, C, a.cp, a.scells, sn, K, px, py, dt, half, sixth,
-                                      a.solout, x, y, kx, ky, o, stop_before);
+            if constexpr (SC != 0)
+                rk4_step_shared<MATH, SC == 2>(B, a.bp, C, a.cp, a.scells, sn, K, px, py, dt,
+                                               half, sixth, a.solout, x, y, kx, ky, o,
+                                               stop_before);
             else
                 rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
                                        a.solout, x, y, kx, ky, o, stop_before);
@@ -333,8 +351,8 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                 double o[4];
                 bool stop_before;
                 if constexpr (SC)
-                    rk4_step_shared<MATH>(B, bp, C, cp, scells, sn, K, px, py, dt, half, sixth,
-                                          a.solout, x, y, kx, ky, o, stop_before);
+                    rk4_step_shared<MATH, false>(B, bp, C, cp, scells, sn, K, px, py, dt, half,
+                                                 sixth, a.solout, x, y, kx, ky, o, stop_before);
                 else
                     rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
                                            a.solout, x, y, kx, ky, o, stop_before);
@@ -475,10 +493,10 @@ __global__ void build_shared_kernel(SharedBuildArgs a)
     const float2 m = cell_basis(a.xp, a.yp, i, j);   // m01 = NaN marks det == 0
     const float2 xq = a.xp[i], yq = a.yp[j];
     r.x0 = xq.x; r.y0 = yq.x; r.m01 = m.x; r.m10 = m.y;
-    r.i1 = (int)i; r.j1 = (int)j;
     const bool err = c >= a.oob_from || __builtin_isnan(m.x);
     if (c >= a.oob_from) {
-        r.a00 = r.a10 = r.a01 = r.a11 = r.hx = r.hy = MR_NANF;
+        r.a00 = r.a10 = r.a01 = r.a11 = MR_NANF;
+        r.hx = r.hy = MR_NAN;
         r.ua00 = r.ua10 = r.ua01 = r.ua11 = MR_NANF;
         r.va00 = r.va10 = r.va01 = r.va11 = MR_NANF;
         r.ux = r.uy = r.vx = r.vy = MR_NAN;
@@ -490,8 +508,9 @@ __global__ void build_shared_kernel(SharedBuildArgs a)
         r.a10 = vB - vA;
         r.a01 = vD - vA;
         r.a11 = ((vC - vA) - r.a10) - r.a01;
-        r.hx = (float)((a.depth[iD] - a.depth[iA]) / a.bxspace);   // cartesian_netcdf3.rs:119-134
-        r.hy = (float)((a.depth[iB] - a.depth[iA]) / a.byspace);
+        // cartesian_netcdf3.rs:119-134: f32 slopes, kept widened to f64
+        r.hx = (double)(float)((a.depth[iD] - a.depth[iA]) / a.bxspace);
+        r.hy = (double)(float)((a.depth[iB] - a.depth[iA]) / a.byspace);
         vA = (float)a.u[iA]; vB = (float)a.u[iB]; vC = (float)a.u[iC]; vD = (float)a.u[iD];
         r.ua00 = vA; r.ua10 = vB - vA; r.ua01 = vD - vA; r.ua11 = ((vC - vA) - r.ua10) - r.ua01;
         vA = (float)a.v[iA]; vB = (float)a.v[iB]; vC = (float)a.v[iC]; vD = (float)a.v[iD];
@@ -522,20 +541,27 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
     const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
                      (a.kyo != nullptr);
     if (a.shared_cells && bk == BK_GRID && ck == CK_GRID) {
-        auto go = [&](auto m) {
-            constexpr int M = decltype(m)::value;
+        // a power-of-two step: dt/2 and dt scale exactly
+        int e2 = 0;
+        const bool dy = std::fabs(std::frexp(a.dt, &e2)) == 0.5 && std::isnormal(a.half);
+        auto go = [&](auto m, auto sc) {
+            constexpr int M = decltype(m)::value, S = decltype(sc)::value;
             if (nout == 0)
-                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 0, true>), grid,
-                                   dim3(block), 0, stream, a);
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 0, S>), grid, dim3(block),
+                                   0, stream, a);
             else if (nout == 4)
-                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 1, true>), grid,
-                                   dim3(block), 0, stream, a);
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 1, S>), grid, dim3(block),
+                                   0, stream, a);
             else
-                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 2, true>), grid,
-                                   dim3(block), 0, stream, a);
+                hipLaunchKernelGGL((trace_kernel<BK_GRID, CK_GRID, M, 2, S>), grid, dim3(block),
+                                   0, stream, a);
         };
-        if (math == MATH_FAST) go(std::integral_constant<int, MATH_FAST>{});
-        else go(std::integral_constant<int, MATH_LITERAL>{});
+        auto by_sc = [&](auto m) {
+            if (dy) go(m, std::integral_constant<int, 2>{});
+            else go(m, std::integral_constant<int, 1>{});
+        };
+        if (math == MATH_FAST) by_sc(std::integral_constant<int, MATH_FAST>{});
+        else by_sc(std::integral_constant<int, MATH_LITERAL>{});
         return hipGetLastError();
     }
     dispatch_kinds(bk, ck, math, [&](auto k) {
