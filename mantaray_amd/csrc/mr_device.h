@@ -500,7 +500,7 @@ struct FastConsts {
     double inv_ln2, ln2_hi, ln2_lo;   // 1/ln 2; ln 2 split in two parts
     double clamp;                     // |argument| limit of exp_pair
     double inv_fact[9];               // 1/11!, 1/10!, ..., 1/3!
-    double g, half_g;                 // g and g/2
+    double g, half_g, m2g;            // g, g/2 and -2g
 };
 
 // e = exp(-r) and em = 1 - exp(-r), each with small relative error, from one
@@ -570,30 +570,36 @@ __device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, d
         // non-finite k*h give all-NaN derivatives in the reference (an
         // error directly; the others through NaN/inf propagation in
         // group_velocity and dkdt_bathy).  Here a provider error poisons h
-        // with NaN, and each of these cases makes cg and c NaN: k = 0 gives
-        // k = 0 * rsq(0) = NaN; h <= 0 gives tanh(kh) <= 0 and a NaN
-        // rsq(g k tanh); a NaN or infinite kh propagates.
+        // with NaN, which exp_pair's clamp (fmax first) turns into
+        // exp(+1400) = inf, so X below is -inf or NaN; h <= 0 makes
+        // X = g k (1-e)(1+e) <= 0 and k = 0 makes k = 0 * rsq(0) NaN: each
+        // gives a NaN W and so NaN derivatives.  A NaN or infinite k*h
+        // propagates through 4kh.
+        //
+        // With e = exp(-2kh) and W = 1/sqrt(g k (1-e)(1+e)), one reciprocal
+        // square root gives every quotient of group_velocity and dkdt_bathy:
+        //   tanh kh = (1-e)/(1+e),  sech^2 kh = 4e/(1+e)^2,
+        //   1/sigma = (1+e) W,      1/(1+e) = g k (1-e) W^2,
+        //   cg = (g/2)(tanh + kh sech^2)/sigma = (g/2) W ((1-e) + 4kh e/(1+e)),
+        //   c = (-k/2) sigma/(sinh kh cosh kh) = -2 g k^2 (e/(1+e)) W,
+        // the last since sinh kh cosh kh = (1-e)(1+e)/(4e).
         h = okc ? h : MR_NAN;
         const double k2 = __builtin_fma(kx, kx, ky * ky);
-        const double rk = rsq_nr(k2);
+        const double rk = rsq_nr(k2);                       // 1/k
         const double k = k2 * rk;
         const double kh = k * h;
-        const double cth = kx * rk, sth = ky * rk;          // cos, sin of atan2(ky, kx)
         double e, em;
-        exp_pair(2.0 * kh, e, em, K);                       // e = exp(-2kh)
+        exp_pair(2.0 * kh, e, em, K);                       // e = exp(-2kh), em = 1 - e
         const double ope = 1.0 + e;
-        const double D = rcp_nr(em * ope);                  // 1/(em (1+e))
-        const double iope = em * D;                         // 1/(1+e)
-        const double T = em * iope;                         // tanh(kh)
-        const double sech2 = (4.0 * e) * (iope * iope);     // 1/cosh^2(kh)
-        const double gkT = (K.g * k) * T;
-        const double rs = rsq_nr(gkT);
-        const double sigma = gkT * rs;                      // sqrt(g k tanh kh)
-        const double cg = K.half_g * __builtin_fma(kh, sech2, T) * rs;
-        // (-k/2)/(sinh cosh) * sigma = -2 k e sigma / (em (1+e))
-        const double c = (-2.0 * k) * e * sigma * D;
-        dx = __builtin_fma(cg, cth, u);
-        dy = __builtin_fma(cg, sth, v);
+        const double gke = (K.g * k) * em;
+        const double W = rsq_nr(gke * ope);
+        const double iope = gke * (W * W);                  // 1/(1+e)
+        const double ei = e * iope;
+        const double cg = (K.half_g * W) * __builtin_fma(4.0 * kh, ei, em);
+        const double cgr = cg * rk;                         // cg cos(theta)/kx = cg/k
+        const double c = (K.m2g * k2) * (ei * W);
+        dx = __builtin_fma(cgr, kx, u);                     // cg cos(atan2(ky, kx)) + u
+        dy = __builtin_fma(cgr, ky, v);
         dkx = (c * hx - kx * ux) - ky * vx;
         dky = (c * hy - kx * uy) - ky * vy;
     }

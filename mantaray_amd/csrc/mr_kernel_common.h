@@ -58,6 +58,7 @@ static __device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
     1.66666666666666667e-01,               // 1/3!
     G,
     0.5 * G,
+    -2.0 * G,
 };
 
 // t[0] = t0, t[s] = t[s-1] + dt for s <= n
