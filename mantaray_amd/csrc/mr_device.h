@@ -499,32 +499,31 @@ __device__ __forceinline__ double rsq_nr(double x)
 struct FastConsts {
     double inv_ln2, ln2_hi, ln2_lo;   // 1/ln 2; ln 2 split in two parts
     double clamp;                     // |argument| limit of exp_pair
-    double inv_fact[9];               // 1/11!, 1/10!, ..., 1/3!
+    double expc[10];                  // expm1(f)/f ~ sum c_j f^j, c_9 first (Horner order)
     double g, half_g, m2g;            // g, g/2 and -2g
+    double nan;                       // a quiet NaN
 };
 
-// e = exp(-r) and em = 1 - exp(-r), each with small relative error, from one
-// reduced-argument evaluation: r = n ln2 + f, |f| <= ln2/2, expm1(-f) by its
-// Taylor series through degree 11 (truncation below 2.2e-14 relative on
-// that interval, near the one-step Newton reciprocals it is combined with),
-// e = 2^-n (1 + p), em = (1 - 2^-n) - 2^-n p, which has no
-// cancellation for n >= 1 and is exactly -p for n = 0.  |r| is clamped to
-// 1400, beyond which exp(-r) is 0 or overflows in f64 either way.
+// e = exp(r) and em = 1 - exp(r) (r <= 0 in use: r = -2kh), each with small
+// relative error, from one reduced-argument evaluation: r = m ln2 + f,
+// |f| <= ln2/2, expm1(f) = f P(f) with P the degree-9 polynomial
+// interpolating expm1(f)/f at the Chebyshev nodes of [-ln2/2, ln2/2] (within
+// 1.8e-15 relative, evaluated in f64), e = 2^m (1 + p), em = (1 - 2^m) -
+// 2^m p, which has no cancellation for m <= -1 and is exactly -p for m = 0.
+// r is clamped to [-1400, 1400], beyond which exp(r) is 0 or overflows in
+// f64 either way; fmin first, so that a NaN r gives exp(+1400) = inf, which
+// makes the callers' results NaN.
 __device__ __forceinline__ void exp_pair(double r, double &e, double &em, const FastConsts &K)
 {
-    r = __builtin_fmin(__builtin_fmax(r, -K.clamp), K.clamp);
-    const double n = __builtin_rint(r * K.inv_ln2);
-    double f = __builtin_fma(-n, K.ln2_hi, r);
-    f = __builtin_fma(-n, K.ln2_lo, f);
-    const double g = -f;
-    // P(g) = sum_{j=0..10} g^j / (j+1)!,  expm1(g) = g * P(g)
-    double P = K.inv_fact[0];
+    r = __builtin_fmax(__builtin_fmin(r, K.clamp), -K.clamp);
+    const double m = __builtin_rint(r * K.inv_ln2);
+    double f = __builtin_fma(-m, K.ln2_hi, r);
+    f = __builtin_fma(-m, K.ln2_lo, f);
+    double P = K.expc[0];
 #pragma unroll
-    for (int j = 1; j < 9; ++j) P = __builtin_fma(P, g, K.inv_fact[j]);
-    P = __builtin_fma(P, g, 0.5);                       // 1/2!
-    P = __builtin_fma(P, g, 1.0);                       // 1/1!
-    const double p = g * P;
-    const double s = __builtin_ldexp(1.0, -(int)n);
+    for (int j = 1; j < 10; ++j) P = __builtin_fma(P, f, K.expc[j]);
+    const double p = f * P;                             // expm1(f)
+    const double s = __builtin_ldexp(1.0, (int)m);
     e = __builtin_fma(s, p, s);
     em = __builtin_fma(-s, p, 1.0 - s);
 }
@@ -570,8 +569,8 @@ __device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, d
         // non-finite k*h give all-NaN derivatives in the reference (an
         // error directly; the others through NaN/inf propagation in
         // group_velocity and dkdt_bathy).  Here a provider error poisons h
-        // with NaN, which exp_pair's clamp (fmax first) turns into
-        // exp(+1400) = inf, so X below is -inf or NaN; h <= 0 makes
+        // with NaN, which exp_pair's clamp turns into exp(+1400) = inf, so
+        // X below is -inf or NaN; h <= 0 makes
         // X = g k (1-e)(1+e) <= 0 and k = 0 makes k = 0 * rsq(0) NaN: each
         // gives a NaN W and so NaN derivatives.  A NaN or infinite k*h
         // propagates through 4kh.
@@ -589,19 +588,20 @@ __device__ __forceinline__ void odes(const FastConsts &K, double h, double hx, d
         const double k = k2 * rk;
         const double kh = k * h;
         double e, em;
-        exp_pair(2.0 * kh, e, em, K);                       // e = exp(-2kh), em = 1 - e
+        exp_pair(-2.0 * kh, e, em, K);                      // e = exp(-2kh), em = 1 - e
         const double ope = 1.0 + e;
         const double gke = (K.g * k) * em;
         const double W = rsq_nr(gke * ope);
         const double iope = gke * (W * W);                  // 1/(1+e)
         const double ei = e * iope;
-        const double cg = (K.half_g * W) * __builtin_fma(4.0 * kh, ei, em);
-        const double cgr = cg * rk;                         // cg cos(theta)/kx = cg/k
+        // cg cos(theta)/kx = cg/k, with (g/2)/k formed beside the chain
+        const double cgr = ((K.half_g * rk) * W) * __builtin_fma(4.0 * kh, ei, em);
         const double c = (K.m2g * k2) * (ei * W);
         dx = __builtin_fma(cgr, kx, u);                     // cg cos(atan2(ky, kx)) + u
         dy = __builtin_fma(cgr, ky, v);
-        dkx = (c * hx - kx * ux) - ky * vx;
-        dky = (c * hy - kx * uy) - ky * vy;
+        // (c hx - kx ux) - ky vx with one rounding per term
+        dkx = __builtin_fma(-ky, vx, __builtin_fma(-kx, ux, c * hx));
+        dky = __builtin_fma(-ky, vy, __builtin_fma(-kx, uy, c * hy));
     }
 }
 
@@ -762,9 +762,12 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
     odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
 }
 
+// all four NaN: fmin returns its non-NaN operand, so the minimum of four is
+// NaN only when all are (the values tested here are results of arithmetic,
+// never signalling NaNs)
 __device__ __forceinline__ bool all_nan4(double a, double b, double c, double d)
 {
-    return __builtin_isnan(a) && __builtin_isnan(b) && __builtin_isnan(c) && __builtin_isnan(d);
+    return __builtin_isnan(__builtin_fmin(__builtin_fmin(a, b), __builtin_fmin(c, d)));
 }
 
 }  // namespace mr

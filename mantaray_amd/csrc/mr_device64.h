@@ -431,7 +431,7 @@ __device__ __forceinline__ void lin_record(const FastConsts &K, const Val64 &h, 
     const double ex = kx * rk, ey = ky * rk;
     const double q = k * hh;
     double e, em;
-    exp_pair(2.0 * q, e, em, K);
+    exp_pair(-2.0 * q, e, em, K);
     const double ope = 1.0 + e;
     const double D = rcp_nr(em * ope);
     const double iope = em * D;
@@ -583,7 +583,7 @@ __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs 
     const double ex = kx * rk, ey = ky * rk;
     const double q = k * hh;
     double e, em;
-    exp_pair(2.0 * q, e, em, K);                        // e = exp(-2q), em = 1 - e
+    exp_pair(-2.0 * q, e, em, K);                        // e = exp(-2q), em = 1 - e
     const double ope = 1.0 + e;
     const double D = rcp_nr(em * ope);
     const double iope = em * D;

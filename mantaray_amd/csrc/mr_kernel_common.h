@@ -47,18 +47,22 @@ static __device__ double mr_fast_consts[sizeof(FastConsts) / sizeof(double)] = {
     6.93147180369123816490e-01,            // ln 2 split, upper part
     1.90821492927058770002e-10,            // ln 2 - upper part
     1400.0,
-    2.50521083854417188e-08,               // 1/11!
-    2.75573192239858907e-07,               // 1/10!
-    2.75573192239858907e-06,               // 1/9!
-    2.48015873015873016e-05,               // 1/8!
-    1.98412698412698413e-04,               // 1/7!
-    1.38888888888888889e-03,               // 1/6!
-    8.33333333333333333e-03,               // 1/5!
-    4.16666666666666667e-02,               // 1/4!
-    1.66666666666666667e-01,               // 1/3!
+    // expm1(f)/f interpolated at the 10 Chebyshev nodes of [-ln2/2, ln2/2]
+    // (computed in 50-digit arithmetic and rounded), c_9 .. c_0
+    2.7620075879983367e-07,
+    2.7632640675430236e-06,
+    2.4801521322368692e-05,
+    1.984119064754424e-04,
+    1.3888888917196719e-03,
+    8.333333367311603e-03,
+    4.1666666666624164e-02,
+    1.6666666666615648e-01,
+    5.000000000000001e-01,
+    1.0000000000000013,
     G,
     0.5 * G,
     -2.0 * G,
+    __builtin_nan(""),
 };
 
 // t[0] = t0, t[s] = t[s-1] + dt for s <= n

@@ -75,7 +75,7 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
             // just pushed: an all-NaN k1 here means the ray already
             // ended on that row.
-            stop_before = solout == 1 && all_nan4(dx, dy, dkx, dky);
+            if (solout == 1) stop_before = all_nan4(dx, dy, dkx, dky);
             ax = dx; ay = dy; akx = dkx; aky = dky;
         } else if (st < 3) {
             // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
@@ -134,7 +134,7 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
         rhs_shared<MATH>(B, BP, C, CP, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
-            stop_before = solout == 1 && all_nan4(dx, dy, dkx, dky);
+            if (solout == 1) stop_before = all_nan4(dx, dy, dkx, dky);
             ax = dx; ay = dy; akx = dkx; aky = dky;
         } else if (st < 3) {
             ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     int64_t s = 1;
     for (; s <= a.n_steps; ++s) {
         if (!__any(alive)) break;
-        double ox = MR_NAN, oy = MR_NAN, okx = MR_NAN, oky = MR_NAN;
+        double ox = K.nan, oy = K.nan, okx = K.nan, oky = K.nan;
         if (alive) {
             double o[4];
             bool stop_before;
