@@ -30,10 +30,13 @@ a HIP graph) against one launch per scene.
 
 Printed: one JSON line with the throughput, the dominant kernel's roofline
 (achieved FP64 rate from its algorithmic work, 745 flop per ray-step, over
-its own event-timed duration; HBM bytes per launch from the rocprofv3
-counter summary in profiles/), and the CPU oracle (OpenMP over rays, all
-host cores) timed on the first 16,384 rays of the same workload (median of
-three runs), with a single-thread figure beside it.
+its own event-timed duration, against the FP64 vector peak: the kernel is
+bound by FP64 vector issue; HBM bytes per launch and record-fetch bytes per
+ray-step from the rocprofv3 counter summary in profiles/, reported only
+while that summary's source hash matches the kernel's sources), the same
+workload with literal math, and the CPU oracle (OpenMP over rays, all host
+cores) timed on the first 16,384 rays of the same workload (median of three
+runs), with a single-thread figure beside it.
 """
 from __future__ import annotations
 
@@ -51,6 +54,8 @@ FLOP_PER_RAY_STEP = 745          # 265 basic FP64 ops + 24 transcendentals x 20
 PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 (vector and matrix)
 # HBM bytes per launch of the trace kernel on this workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size
+# (mantaray_amd.profiling); used only while its source hash matches the
+# kernel's sources
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_kernel_seamount_jet_traffic.json")
 
 
@@ -69,6 +74,8 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     p.add_argument("--no-inversion", action="store_true", help="skip the inversion run")
     p.add_argument("--no-batch", action="store_true", help="skip the batched-scenes run")
+    p.add_argument("--no-literal", action="store_true",
+                   help="skip the literal-math launches of the same workload")
     p.add_argument("--cpu-rays", type=int, default=16384,
                    help="rays of the workload the CPU oracle traces (from the first)")
     return p.parse_args()
@@ -378,10 +385,34 @@ def main():
 
     value = ray_steps * args.steps / wall
     achieved = ray_steps * FLOP_PER_RAY_STEP / (kern_ms * 1e-3) / 1e12
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    from mantaray_amd.profiling import load_traffic, source_hash
+    tfile = load_traffic(TRAFFIC_FILE) if args.math == "fast" else None
+    traffic = None if tfile is None else tfile["hbm_bytes_per_launch"]
+
+    # the same launches with literal math (libm-style exp, sqrt and divide
+    # in place of the fast dispersion form), beside the headline
+    literal = None
+    if not args.no_literal and args.math == "fast":
+        def run_lit():
+            core.trace_many_device(b, c, *ic, cfg["t0"], cfg["t_end"], cfg["dt"],
+                                   math="literal", block_size=args.block, out=out,
+                                   stream=stream, record_tile=tile)
+        run_lit()
+        lit_steps = int((out["len"].to(torch.int64) - 1).sum().item())
+        reps = max(2, args.steps // 4)
+        lev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        torch.cuda.synchronize()
+        for e0, e1 in lev:
+            e0.record(stream)
+            run_lit()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        lit_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in lev]))
+        literal = {"ray_steps_per_s": lit_steps / (lit_ms * 1e-3), "kernel_ms": lit_ms,
+                   "launches": reps, "ray_steps_per_launch": lit_steps,
+                   "frac_at_745_flop": lit_steps * FLOP_PER_RAY_STEP / (lit_ms * 1e-3) / 1e12
+                   / PEAK_FP64_TFLOPS}
     res = {
         "metric": "FP64 RK ray-steps/s on 1 MI355X vs host CPU; adjoint fwd+bwd / fwd cost",
         "value": value,
@@ -401,13 +432,22 @@ def main():
                    "math": args.math, "parallelism": "rays, one per lane",
                    "record_tile": tile,
                    "record_tile_table_ms": [[ct["record_tile"], t] for ct, t in tile_table]},
-        "roofline": {"bound": "mfma", "roof": "FP64 (vector and matrix share the peak)",
+        "roofline": {"bound": "fp64-valu",
+                     "roof": "FP64 vector issue (no MFMA: one ray per lane, 78.6 TFLOP/s is "
+                             "the MI355X FP64 vector peak)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel_ms": kern_ms,
                      "work": f"{FLOP_PER_RAY_STEP} flop/ray-step x {ray_steps} ray-steps",
                      "hbm_GBps": None if traffic is None else traffic / (kern_ms * 1e-3) / 1e9,
-                     "hbm_peak_GBps": 8000.0},
+                     "hbm_peak_GBps": 8000.0,
+                     "record_fetch_B_per_ray_step": None if tfile is None
+                     else tfile["fetch_bytes_per_ray_step"],
+                     "traffic_source_hash": source_hash(),
+                     "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                                     "passes scaled by a 1 GiB copy; null when they were not "
+                                     "measured on this kernel's sources"},
+        "literal_math": literal,
     }
     # the trajectories of the forward measurement (10.5 GB) are not needed by
     # the adjoint and inversion measurements, whose memory use is reported

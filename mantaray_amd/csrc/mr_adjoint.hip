@@ -166,7 +166,9 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
             double k[3][4], n[4];
             rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half, sixth, s,
                                      k, n);
-            if (a.solout == 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
+            // the MR_SOLOUT_NEXT_DERIV test applies from the second step on
+            // (solout is asked only after a step)
+            if (a.solout == 1 && s_i > 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
                 alive = false;         // the ray ended on the row before
             } else {
                 o[0] = n[0]; o[1] = n[1]; o[2] = n[2]; o[3] = n[3];

@@ -765,9 +765,14 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
 // all four NaN: fmin returns its non-NaN operand, so the minimum of four is
 // NaN only when all are (the values tested here are results of arithmetic,
 // never signalling NaNs)
+__device__ __forceinline__ double min4(double a, double b, double c, double d)
+{
+    return __builtin_fmin(__builtin_fmin(a, b), __builtin_fmin(c, d));
+}
+
 __device__ __forceinline__ bool all_nan4(double a, double b, double c, double d)
 {
-    return __builtin_isnan(__builtin_fmin(__builtin_fmin(a, b), __builtin_fmin(c, d)));
+    return __builtin_isnan(min4(a, b, c, d));
 }
 
 }  // namespace mr

@@ -9,6 +9,14 @@
 
 namespace mr {
 
+// Word 3 of a raw buffer descriptor on the gfx9 family (gfx950 among them):
+// DATA_FORMAT (bits 18:15) = 4, 32-bit, a valid format for untyped
+// accesses; every other field zero.
+constexpr int MR_BUFFER_DWORD3 = 0x00020000;
+// cache-policy operand of the buffer intrinsics: bit 1 is the nontemporal
+// (streaming) hint on gfx950
+constexpr int MR_CPOL_NT = 2;
+
 // Copy a field view from device memory into per-lane registers.  The loads
 // go through a lane-varying (always zero) offset so they are vector loads
 // and the view lives in VGPRs for the whole kernel: the scalar register file

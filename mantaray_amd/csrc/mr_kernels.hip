@@ -33,16 +33,20 @@ namespace mr {
 
 // One RK4 step of one lane from (x, y, kx, ky): the new state in o, and
 // stop_before when (MR_SOLOUT_NEXT_DERIV) the first stage derivative is
-// all-NaN, i.e. the ray already ended on the row it starts from.  bn, cn:
+// all-NaN, i.e. the ray already ended on the row it starts from; on_stop()
+// is then called after the first stage, while (x, y, kx, ky) is still the
+// state the step started from.  Every stage state after an all-NaN first
+// derivative is NaN, and so is every later derivative and the whole of o.
+// bn, cn:
 // the records prefetched for the next evaluation; (px, py): the latest
 // dx/dt, dy/dt, for the guess.
-template <int BK, int CK, int MATH>
+template <int BK, int CK, int MATH, class OnStop>
 __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP, BathySlot &bn,
                                          const CurrentView &C, const CurrentPtrs &CP,
                                          CurrentSlot &cn, const FastConsts &K, double &px,
                                          double &py, double dt, double half, double sixth,
                                          int solout, double x, double y, double kx, double ky,
-                                         double o[4], bool &stop_before)
+                                         double o[4], bool &stop_before, OnStop &&on_stop)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
     double sx = x, sy = y, skx = kx, sky = ky;   // stage state
@@ -75,7 +79,10 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
             // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
             // just pushed: an all-NaN k1 here means the ray already
             // ended on that row.
-            if (solout == 1) stop_before = all_nan4(dx, dy, dkx, dky);
+            if (solout == 1) {
+                stop_before = all_nan4(dx, dy, dkx, dky);
+                if (stop_before) on_stop();
+            }
             ax = dx; ay = dy; akx = dkx; aky = dky;
         } else if (st < 3) {
             // k * 2 is exact, so fma(k, 2, a) rounds as a + k * 2
@@ -101,14 +108,14 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
 // DY: dt and dt/2 are powers of two, so k * hs is exact (for every k whose
 // product is not subnormal) and the stage state y + k * hs of Rk4::step is
 // one rounding: fma(k, hs, y) gives it bitwise in one instruction.
-template <int MATH, bool DY>
+template <int MATH, bool DY, class OnStop>
 __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyPtrs &BP,
                                                 const CurrentView &C, const CurrentPtrs &CP,
                                                 const SCell *cells, SharedSlot &sn,
                                                 const FastConsts &K, double &px, double &py,
                                                 double dt, double half, double sixth, int solout,
                                                 double x, double y, double kx, double ky,
-                                                double o[4], bool &stop_before)
+                                                double o[4], bool &stop_before, OnStop &&on_stop)
 {
     double sx = x, sy = y, skx = kx, sky = ky;   // stage state
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
@@ -134,7 +141,10 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
         rhs_shared<MATH>(B, BP, C, CP, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
-            if (solout == 1) stop_before = all_nan4(dx, dy, dkx, dky);
+            if (solout == 1) {
+                stop_before = all_nan4(dx, dy, dkx, dky);
+                if (stop_before) on_stop();
+            }
             ax = dx; ay = dy; akx = dkx; aky = dky;
         } else if (st < 3) {
             ax = __builtin_fma(dx, 2.0, ax); ay = __builtin_fma(dy, 2.0, ay);
@@ -161,6 +171,26 @@ __device__ __forceinline__ void shared_prefetch(const BathyView &B, const SCell 
     const int i1 = guess_cell((float)x, B.ax0, B.pinv_sx, B.pmax_x);
     const int j1 = guess_cell((float)y, B.ay0, B.pinv_sy, B.pmax_y);
     shared_load(cells, s, tiled_cell(i1, j1, B.ntx));
+}
+
+// A buffer descriptor over the rows from p on (raw: no stride, no swizzle;
+// the record count is 2^32 - 1 - k, so no offset below 2^31 is cut off).
+// k tells the four descriptors of a trace apart: with equal upper words the
+// compiler keeps one copy of those words and re-copies it next to each
+// descriptor at every step.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(double *p, int k)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, -1 - k, MR_BUFFER_DWORD3);
+}
+
+// one double at byte offset voff + soff of the descriptor's memory, with the
+// nontemporal hint
+__device__ __forceinline__ void store_row_nt(__amdgpu_buffer_rsrc_t d, uint32_t voff,
+                                             uint32_t soff, double v)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), d, (int)voff, (int)soff,
+                                          MR_CPOL_NT);
 }
 
 // Lanes past n_rays repeat the last ray (same inputs, same arithmetic, so
@@ -215,44 +245,98 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         if (PF_C) settle(cn);
     }
     double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
-    bool alive = true;
-    int64_t len = 1;
+    // A lane's ray is live while its state is not all-NaN.  The step that
+    // ends a ray leaves its state all-NaN (an all-NaN new state, or an
+    // all-NaN first stage derivative, which makes every component of the new
+    // state NaN), so liveness is read off the state at each step, the row
+    // stored is the state itself, and no per-lane flag is carried from step
+    // to step (a carried flag costs mask merges at every join).  len counts
+    // the rows; a ray that the MR_SOLOUT_NEXT_DERIV test stops before a step
+    // ends on the state that step started from, written to end_out at once,
+    // and is marked by a negative len.
+    int32_t len = 1;
+    // The integration loop asks solout only after a step, so every ray takes
+    // its first step, and the MR_SOLOUT_NEXT_DERIV test (f at the state a
+    // step starts from) applies from the second step on: so carries the
+    // solout variant into each step, none for the first.
+    int so = 0;
+    // A ray that starts all-NaN takes that first step in the reference; its
+    // row is NaN (every derivative at an all-NaN state is NaN) and counts,
+    // and the ray ends there.  Such a lane takes no step here and starts
+    // with the row count that step leaves.
+    if (a.n_steps > 0 && all_nan4(x, y, kx, ky)) len = 2;
+    // min4 of the state, NaN exactly when the state is all-NaN: formed after
+    // each step from the step's arithmetic, where the compiler knows no
+    // signalling NaN can occur (at the loop head it would quiet all four
+    // components first)
+    double m = min4(x, y, kx, ky);
+    // OUT == 1: the rows go out by buffer stores from one descriptor per
+    // component, based at the first row of a block of rows; the row's
+    // offset in the block is one scalar register shared by the four stores
+    // and the lane's offset one constant vector register.  The launcher
+    // keeps a row under 2^31 bytes, and a block's offsets stay below 2^31.
+    const uint32_t row_bytes = OUT == 1 ? (uint32_t)nr * 8u : 1u;
+    const uint32_t lane_off = (uint32_t)r * 8u;
+    const int64_t block_rows =
+        OUT == 1 ? __builtin_elementwise_max((int64_t)((0x7fffffffu - row_bytes) / row_bytes),
+                                             (int64_t)1)
+                 : (int64_t)1 << 30;
     int64_t s = 1;
-    for (; s <= a.n_steps; ++s) {
-        if (!__any(alive)) break;
-        double ox = K.nan, oy = K.nan, okx = K.nan, oky = K.nan;
-        if (alive) {
-            double o[4];
-            bool stop_before;
-            if constexpr (SC != 0)
-                rk4_step_shared<MATH, SC == 2>(B, a.bp, C, a.cp, a.scells, sn, K, px, py, dt,
-                                               half, sixth, a.solout, x, y, kx, ky, o,
-                                               stop_before);
-            else
-                rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half, sixth,
-                                       a.solout, x, y, kx, ky, o, stop_before);
-            if (stop_before) {
-                alive = false;
-            } else {
-                ox = o[0]; oy = o[1]; okx = o[2]; oky = o[3];
-                x = ox; y = oy; kx = okx; ky = oky;
-                len = s + 1;
-                if (all_nan4(ox, oy, okx, oky)) alive = false;
+    bool more = true;
+    while (more && s <= a.n_steps) {
+        const int64_t s_end = __builtin_elementwise_min(a.n_steps, s + block_rows - 1);
+        const int64_t row0 = OUT == 1 ? s * nr : 0;
+        const __amdgpu_buffer_rsrc_t bx = row_rsrc(a.xo + row0, 0), by = row_rsrc(a.yo + row0, 1),
+                                     bkx = row_rsrc(a.kxo + row0, 2),
+                                     bky = row_rsrc(a.kyo + row0, 3);
+        uint32_t soff = 0;
+        // the block's steps counted down in 32 bits: the scalar unit tests a
+        // 32-bit count in one instruction, a 64-bit signed bound only through
+        // a vector compare
+        for (uint32_t left = (uint32_t)(s_end - s + 1); left != 0u;
+             --left, ++s, soff += row_bytes) {
+            const bool live = !__builtin_isnan(m);
+            if (__builtin_amdgcn_ballot_w64(live) == 0) {
+                more = false;
+                break;
             }
-        }
-        const int64_t o = s * nr + r;
-        if (OUT == 1) {
-            // streaming stores: the rows are not read again by the kernel,
-            // and keeping them out of L2 leaves it to the cell records
-            __builtin_nontemporal_store(ox, a.xo + o);
-            __builtin_nontemporal_store(oy, a.yo + o);
-            __builtin_nontemporal_store(okx, a.kxo + o);
-            __builtin_nontemporal_store(oky, a.kyo + o);
-        } else if (OUT == 2) {
-            if (a.xo) a.xo[o] = ox;
-            if (a.yo) a.yo[o] = oy;
-            if (a.kxo) a.kxo[o] = okx;
-            if (a.kyo) a.kyo[o] = oky;
+            if (live) {
+                double o[4];
+                bool stop_before;
+                auto on_stop = [&] {
+                    if (a.end_out) {
+                        a.end_out[4 * r + 0] = x; a.end_out[4 * r + 1] = y;
+                        a.end_out[4 * r + 2] = kx; a.end_out[4 * r + 3] = ky;
+                    }
+                    len = -(len + 1);
+                };
+                if constexpr (SC != 0)
+                    rk4_step_shared<MATH, SC == 2>(B, a.bp, C, a.cp, a.scells, sn, K, px, py,
+                                                   dt, half, sixth, so, x, y, kx, ky, o,
+                                                   stop_before, on_stop);
+                else
+                    rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half,
+                                           sixth, so, x, y, kx, ky, o, stop_before, on_stop);
+                x = o[0]; y = o[1]; kx = o[2]; ky = o[3];
+                m = min4(x, y, kx, ky);
+                ++len;
+            }
+            if (OUT == 1) {
+                // streaming stores: the rows are not read again by the
+                // kernel, and keeping them out of L2 leaves it to the cell
+                // records
+                store_row_nt(bx, lane_off, soff, x);
+                store_row_nt(by, lane_off, soff, y);
+                store_row_nt(bkx, lane_off, soff, kx);
+                store_row_nt(bky, lane_off, soff, ky);
+            } else if (OUT == 2) {
+                const int64_t o = s * nr + r;
+                if (a.xo) a.xo[o] = x;
+                if (a.yo) a.yo[o] = y;
+                if (a.kxo) a.kxo[o] = kx;
+                if (a.kyo) a.kyo[o] = ky;
+            }
+            so = a.solout;
         }
     }
     // rows past the wave's last live step
@@ -265,8 +349,8 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
             if (OUT == 1 || a.kyo) __builtin_nontemporal_store(MR_NAN, a.kyo + o);
         }
     }
-    if (a.len_out) a.len_out[r] = (int32_t)len;
-    if (a.end_out) {
+    if (a.len_out) a.len_out[r] = len < 0 ? -len : len;
+    if (a.end_out && len > 0) {
         a.end_out[4 * r + 0] = x; a.end_out[4 * r + 1] = y;
         a.end_out[4 * r + 2] = kx; a.end_out[4 * r + 3] = ky;
     }
@@ -352,12 +436,17 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                 bool stop_before;
                 if constexpr (SC)
                     rk4_step_shared<MATH, false>(B, bp, C, cp, scells, sn, K, px, py, dt, half,
-                                                 sixth, a.solout, x, y, kx, ky, o, stop_before);
+                                                 sixth, a.solout, x, y, kx, ky, o, stop_before,
+                                                 [] {});
                 else
                     rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
-                                           a.solout, x, y, kx, ky, o, stop_before);
+                                           a.solout, x, y, kx, ky, o, stop_before, [] {});
                 ++step;
-                if (stop_before) {
+                // the MR_SOLOUT_NEXT_DERIV test applies from a ray's second
+                // step on (solout is asked only after a step); at the first,
+                // an all-NaN k1 makes the new state all-NaN, ending the ray
+                // one row later
+                if (stop_before && step > 1) {
                     finished = true;
                 } else {
                     x = o[0]; y = o[1]; kx = o[2]; ky = o[3];
@@ -538,8 +627,11 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
     if (a.n_rays <= 0) return hipSuccess;
     if (a.ray_blocks != (int)((a.n_rays + block - 1) / block)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
-    const int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) +
-                     (a.kyo != nullptr);
+    // all four components go out by buffer stores (trace_kernel's OUT 1)
+    // while one row stays under 2^31 bytes; otherwise by the kernel that
+    // takes any subset (OUT 2)
+    int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) + (a.kyo != nullptr);
+    if (nout == 4 && a.n_rays >= (int64_t(1) << 28)) nout = 5;
     if (a.shared_cells && bk == BK_GRID && ck == CK_GRID) {
         // a power-of-two step: dt/2 and dt scale exactly
         int e2 = 0;
