@@ -17,6 +17,17 @@ constexpr int MR_BUFFER_DWORD3 = 0x00020000;
 // (streaming) hint on gfx950
 constexpr int MR_CPOL_NT = 2;
 
+// s_waitcnt on the vector memory count alone: wait until at most N vector
+// memory operations are outstanding (gfx9 encoding: vmcnt in bits 3:0,
+// expcnt 6:4 and lgkmcnt 11:8 at their no-wait maxima, vmcnt bits 15:14
+// zero).
+template <int N>
+__device__ __forceinline__ void vm_wait()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt below 16");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
+
 // Copy a field view from device memory into per-lane registers.  The loads
 // go through a lane-varying (always zero) offset so they are vector loads
 // and the view lives in VGPRs for the whole kernel: the scalar register file

@@ -99,46 +99,61 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
 }
 
 // rk4_step for gridded depth and current on shared cells: one combined
-// record (SCell) per cell serves both fields.  sn: the record prefetched for
-// the next evaluation, loaded at stages 1 and 3 when the guessed point of the
-// pair that follows lies outside the fast region of the record in hand (a
-// prefetch is only a guess: rhs_shared tests the region it uses, and lanes
-// outside it take the full queries).
+// record (SCell) per cell serves both fields, from two slots with fixed
+// roles.  In the stage loop below (st = 0..3), sa serves st 0 and st 3 (the
+// step's start y and y + k3 dt, nearly the next step's start), sb serves
+// st 1 and st 2 (y + k1 dt/2 and y + k2 dt/2).  Each slot is refilled while
+// the other one serves, two evaluations before its next use: sa at st 1 for
+// the point guessed at x + k1 dt, sb at st 3 for the next step's pair,
+// guessed at x + k3 (3 dt / 2).  A lane loads only when the guess lies
+// outside the fast region of the record its slot holds, and otherwise keeps
+// that record, the right one; so no slot is ever copied.  A guess is only a
+// guess: rhs_shared tests the region of the record it uses, and lanes
+// outside it take the full queries.  g1, g3: dt and 3 dt / 2 in f32.
+//
+// Before each refill the step waits for the vector memory operations issued
+// up to then but the last NST (the caller's row stores of the step before,
+// issued after every load): the serving slot's loads, two evaluations old,
+// have landed by then.  Without that wait the compiler waits at the serving
+// slot's first use, after the refill, and as the refill is conditional it
+// cannot count it: it waits for the refill's loads as well.
 //
 // DY: dt and dt/2 are powers of two, so k * hs is exact (for every k whose
 // product is not subnormal) and the stage state y + k * hs of Rk4::step is
 // one rounding: fma(k, hs, y) gives it bitwise in one instruction.
-template <int MATH, bool DY, class OnStop>
+template <int MATH, bool DY, int NST, class OnStop>
 __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyPtrs &BP,
                                                 const CurrentView &C, const CurrentPtrs &CP,
-                                                const SCell *cells, SharedSlot &sn,
-                                                const FastConsts &K, double &px, double &py,
-                                                double dt, double half, double sixth, int solout,
+                                                const SCell *cells, SharedSlot &sa,
+                                                SharedSlot &sb, const FastConsts &K, double &px,
+                                                double &py, double dt, double half,
+                                                double sixth, float g1, float g3, int solout,
                                                 double x, double y, double kx, double ky,
                                                 double o[4], bool &stop_before, OnStop &&on_stop)
 {
     double sx = x, sy = y, skx = kx, sky = ky;   // stage state
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
     stop_before = false;
-    SharedSlot sq = sn;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const double hs = st < 2 ? half : dt;
         if (st == 1 || st == 3) {
-            sq = sn;
-        } else {
+            if (st == 1) vm_wait<NST>();
+            else vm_wait<0>();
             // the guessed point in f32: it only chooses which record to load
-            const float fh = (float)hs;
-            const float gx = __builtin_fmaf((float)px, fh, (float)x);
-            const float gy = __builtin_fmaf((float)py, fh, (float)y);
-            if (!shared_in_region(sq, gx, gy)) {
+            SharedSlot &refill = st == 1 ? sa : sb;
+            const float g = st == 1 ? g1 : g3;
+            const float gx = __builtin_fmaf((float)px, g, (float)x);
+            const float gy = __builtin_fmaf((float)py, g, (float)y);
+            if (!shared_in_region(refill, gx, gy)) {
                 const int i1 = guess_cell(gx, B.ax0, B.pinv_sx, B.pmax_x);
                 const int j1 = guess_cell(gy, B.ay0, B.pinv_sy, B.pmax_y);
-                shared_load(cells, sn, tiled_cell(i1, j1, B.ntx));
+                shared_load(cells, refill, tiled_cell(i1, j1, B.ntx));
             }
         }
         double dx, dy, dkx, dky;
-        rhs_shared<MATH>(B, BP, C, CP, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        rhs_shared<MATH>(B, BP, C, CP, (st == 0 || st == 3) ? sa : sb, K, sx, sy, skx, sky, dx,
+                         dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
             if (solout == 1) {
@@ -164,13 +179,16 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
     o[2] = kx + akx * sixth; o[3] = ky + aky * sixth;
 }
 
-// the first record of a lane: the cell its start most likely lies in
+// the first records of a lane: both slots hold the cell its start most
+// likely lies in
 __device__ __forceinline__ void shared_prefetch(const BathyView &B, const SCell *cells,
-                                                SharedSlot &s, double x, double y)
+                                                SharedSlot &sa, SharedSlot &sb, double x,
+                                                double y)
 {
     const int i1 = guess_cell((float)x, B.ax0, B.pinv_sx, B.pmax_x);
     const int j1 = guess_cell((float)y, B.ay0, B.pinv_sy, B.pmax_y);
-    shared_load(cells, s, tiled_cell(i1, j1, B.ntx));
+    shared_load(cells, sa, tiled_cell(i1, j1, B.ntx));
+    shared_load(cells, sb, tiled_cell(i1, j1, B.ntx));
 }
 
 // A buffer descriptor over the rows from p on (raw: no stride, no swizzle;
@@ -234,10 +252,12 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     settle(K);
     BathySlot bn;                      // records prefetched for the next evaluation
     CurrentSlot cn;
-    SharedSlot sn;                     // the same, for shared cells
+    SharedSlot sa, sb;                 // the two record slots, for shared cells
+    const float g1 = (float)dt, g3 = (float)(dt + half);
     if (SC) {
-        shared_prefetch(B, a.scells, sn, x, y);
-        settle(sn);                    // likewise for the first record
+        shared_prefetch(B, a.scells, sa, sb, x, y);
+        settle(sa);                    // likewise for the first records
+        settle(sb);
     } else {
         if (PF_B) bathy_prefetch(B, a.bp, bn, x, y);
         if (PF_C) current_prefetch(C, a.cp, cn, x, y);
@@ -311,9 +331,9 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                     len = -(len + 1);
                 };
                 if constexpr (SC != 0)
-                    rk4_step_shared<MATH, SC == 2>(B, a.bp, C, a.cp, a.scells, sn, K, px, py,
-                                                   dt, half, sixth, so, x, y, kx, ky, o,
-                                                   stop_before, on_stop);
+                    rk4_step_shared<MATH, SC == 2, OUT == 1 ? 4 : 0>(
+                        B, a.bp, C, a.cp, a.scells, sa, sb, K, px, py, dt, half, sixth, g1, g3,
+                        so, x, y, kx, ky, o, stop_before, on_stop);
                 else
                     rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half,
                                            sixth, so, x, y, kx, ky, o, stop_before, on_stop);
@@ -379,7 +399,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
     CurrentPtrs cp{};
     BathySlot bn;
     CurrentSlot cn;
-    SharedSlot sn;
+    SharedSlot sa, sb;
     const SCell *scells = nullptr;
     double x = 0.0, y = 0.0, kx = 0.0, ky = 0.0, px = 0.0, py = 0.0;
     double dt = 0.0, half = 0.0, sixth = 0.0;
@@ -415,7 +435,7 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                     step = 0; len = 1; px = 0.0; py = 0.0;
                     if (SC) {
                         scells = S->scells;
-                        shared_prefetch(B, scells, sn, x, y);
+                        shared_prefetch(B, scells, sa, sb, x, y);
                     } else {
                         bn = BathySlot{};
                         cn = CurrentSlot{};
@@ -435,9 +455,9 @@ __global__ void __launch_bounds__(256) batch_kernel(BatchArgs a)
                 double o[4];
                 bool stop_before;
                 if constexpr (SC)
-                    rk4_step_shared<MATH, false>(B, bp, C, cp, scells, sn, K, px, py, dt, half,
-                                                 sixth, a.solout, x, y, kx, ky, o, stop_before,
-                                                 [] {});
+                    rk4_step_shared<MATH, false, 0>(B, bp, C, cp, scells, sa, sb, K, px, py, dt,
+                                                 half, sixth, (float)dt, (float)(dt + half),
+                                                 a.solout, x, y, kx, ky, o, stop_before, [] {});
                 else
                     rk4_step<BK, CK, MATH>(B, bp, bn, C, cp, cn, K, px, py, dt, half, sixth,
                                            a.solout, x, y, kx, ky, o, stop_before, [] {});
