@@ -216,12 +216,14 @@ __device__ __forceinline__ void store_row_nt(__amdgpu_buffer_rsrc_t d, uint32_t 
 // stores every row, with no branch around the stores.  OUT: 0 no
 // trajectories, 1 all four components, 2 any subset.
 // SC: 0 separate records per field; 1 the shared-cell records (SCell) of a
-// depth and current on the same cells; 2 the same with dt a power of two
-// (rk4_step_shared's DY).
+// depth and current on the same cells (rk4_step_shared); 2 the same with dt
+// a power of two (its DY).
 template <int BK, int CK, int MATH, int OUT, int SC = 0>
 __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
 {
     constexpr bool PF_B = BK == BK_GRID, PF_C = CK == CK_GRID;
+    constexpr bool DYAD = SC == 2;
+    constexpr int NST = OUT == 1 ? 4 : 0;   // row stores issued per step
     if ((int)blockIdx.x == a.ray_blocks) {
         // the time column: t_s = t_{s-1} + dt (ode_solvers accumulates), a
         // sequential rounding chain, run by one lane beside the rays
@@ -294,7 +296,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
     // component, based at the first row of a block of rows; the row's
     // offset in the block is one scalar register shared by the four stores
     // and the lane's offset one constant vector register.  The launcher
-    // keeps a row under 2^31 bytes, and a block's offsets stay below 2^31.
+    // keeps a row under 2^30 bytes, and a block's offsets stay below 2^31.
     const uint32_t row_bytes = OUT == 1 ? (uint32_t)nr * 8u : 1u;
     const uint32_t lane_off = (uint32_t)r * 8u;
     const int64_t block_rows =
@@ -302,8 +304,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                                              (int64_t)1)
                  : (int64_t)1 << 30;
     int64_t s = 1;
-    bool more = true;
-    while (more && s <= a.n_steps) {
+    while (s <= a.n_steps) {
         const int64_t s_end = __builtin_elementwise_min(a.n_steps, s + block_rows - 1);
         const int64_t row0 = OUT == 1 ? s * nr : 0;
         const __amdgpu_buffer_rsrc_t bx = row_rsrc(a.xo + row0, 0), by = row_rsrc(a.yo + row0, 1),
@@ -312,14 +313,13 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
         uint32_t soff = 0;
         // the block's steps counted down in 32 bits: the scalar unit tests a
         // 32-bit count in one instruction, a 64-bit signed bound only through
-        // a vector compare
+        // a vector compare.  The loops are left by goto when every lane's ray
+        // has ended: a flag tested by the block loop would keep, in the
+        // compiler's view, a path from that exit back into the step loop.
         for (uint32_t left = (uint32_t)(s_end - s + 1); left != 0u;
              --left, ++s, soff += row_bytes) {
             const bool live = !__builtin_isnan(m);
-            if (__builtin_amdgcn_ballot_w64(live) == 0) {
-                more = false;
-                break;
-            }
+            if (__builtin_amdgcn_ballot_w64(live) == 0) goto done;
             if (live) {
                 double o[4];
                 bool stop_before;
@@ -331,9 +331,9 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
                     len = -(len + 1);
                 };
                 if constexpr (SC != 0)
-                    rk4_step_shared<MATH, SC == 2, OUT == 1 ? 4 : 0>(
-                        B, a.bp, C, a.cp, a.scells, sa, sb, K, px, py, dt, half, sixth, g1, g3,
-                        so, x, y, kx, ky, o, stop_before, on_stop);
+                    rk4_step_shared<MATH, DYAD, NST>(B, a.bp, C, a.cp, a.scells, sa, sb, K, px,
+                                                     py, dt, half, sixth, g1, g3, so, x, y, kx,
+                                                     ky, o, stop_before, on_stop);
                 else
                     rk4_step<BK, CK, MATH>(B, a.bp, bn, C, a.cp, cn, K, px, py, dt, half,
                                            sixth, so, x, y, kx, ky, o, stop_before, on_stop);
@@ -359,6 +359,7 @@ __global__ void __launch_bounds__(256) trace_kernel(TraceArgs a)
             so = a.solout;
         }
     }
+done:
     // rows past the wave's last live step
     if (OUT != 0) {
         for (; s <= a.n_steps; ++s) {
@@ -648,10 +649,10 @@ hipError_t launch_trace(const TraceArgs &a, int bk, int ck, int math, int block,
     if (a.ray_blocks != (int)((a.n_rays + block - 1) / block)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(a.ray_blocks + (a.t_out ? 1 : 0)));
     // all four components go out by buffer stores (trace_kernel's OUT 1)
-    // while one row stays under 2^31 bytes; otherwise by the kernel that
+    // while one row stays under 2^30 bytes; otherwise by the kernel that
     // takes any subset (OUT 2)
     int nout = (a.xo != nullptr) + (a.yo != nullptr) + (a.kxo != nullptr) + (a.kyo != nullptr);
-    if (nout == 4 && a.n_rays >= (int64_t(1) << 28)) nout = 5;
+    if (nout == 4 && a.n_rays >= (int64_t(1) << 27)) nout = 5;
     if (a.shared_cells && bk == BK_GRID && ck == CK_GRID) {
         // a power-of-two step: dt/2 and dt scale exactly
         int e2 = 0;
