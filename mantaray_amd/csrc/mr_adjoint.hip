@@ -564,26 +564,14 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
     const int cmax = wave_max(cmax_r);
     const int rounds = cmax >= 0 ? (cmax + 2) * stride + 1 : 0;
     const int chain = NL + 1;
-    Slots64 sn;                                        // cells for the next evaluation
-    double s[4] = {0.0, 0.0, 0.0, 0.0};                // wave 0: the state being advanced
-    double px = 0.0, py = 0.0;
-    double nxt[16];                                    // linearising waves: the next step's data
-    bool pre = false;
-    double lam[4] = {0.0, 0.0, 0.0, 0.0};              // chain wave: the cotangent
-    Accs64 acc;
-    if (wave == chain && active && m >= 0) {
-        if (a.g_end && !dead) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
-        }
-        const int64_t o = m * nr + r;
-        if (a.g_x) lam[0] += a.g_x[o];
-        if (a.g_y) lam[1] += a.g_y[o];
-        if (a.g_kx) lam[2] += a.g_kx[o];
-        if (a.g_ky) lam[3] += a.g_ky[o];
-    }
-    for (int rho = 0; rho < rounds; ++rho) {
-        if (wave == 0) {
+    // Each role runs a loop of its own, with the same rounds and one barrier
+    // per round: the state of one role is then not live across another's
+    // code, and registers go to the largest role, not to all of them.
+    if (wave == 0) {
+        Slots64 sn;                                    // cells for the next evaluation
+        double s[4] = {0.0, 0.0, 0.0, 0.0};            // the state being advanced
+        double px = 0.0, py = 0.0;
+        for (int rho = 0; rho < rounds; ++rho) {
             const int c = cmax - rho / stride, j = rho % stride;
             if (c >= 0 && c <= cmax_r) {
                 const int64_t i0 = (int64_t)c * stride;
@@ -608,7 +596,13 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     }
                 }
             }
-        } else if (wave < chain) {
+            __syncthreads();
+        }
+    } else if (wave < chain) {
+        Slots64 sn;
+        double nxt[16];                                // the next step's data
+        bool pre = false;
+        for (int rho = 0; rho < rounds; ++rho) {
             const int qq = rho / stride;
             const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
             if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
@@ -712,50 +706,68 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
             } else {
                 pre = false;
             }
-        } else if (rho >= 1) {
-            const int q1 = (rho - 1) / stride;
-            const int c = cmax - q1 + 1, j = stride - 1 - (rho - 1) % stride;
-            if (q1 >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
-                double kb[4][4];
+            __syncthreads();
+        }
+    } else {
+        double lam[4] = {0.0, 0.0, 0.0, 0.0};          // the cotangent
+        Accs64 acc;
+        if (active && m >= 0) {
+            if (a.g_end && !dead) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double w = lam[q] * sixth;
-                    kb[0][q] = w; kb[3][q] = w;
-                    kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
-                }
+                for (int q = 0; q < 4; ++q) lam[q] = a.g_end[4 * r + q];
+            }
+            const int64_t o = m * nr + r;
+            if (a.g_x) lam[0] += a.g_x[o];
+            if (a.g_y) lam[1] += a.g_y[o];
+            if (a.g_kx) lam[2] += a.g_kx[o];
+            if (a.g_ky) lam[3] += a.g_ky[o];
+        }
+        for (int rho = 0; rho < rounds; ++rho) {
+            if (rho >= 1) {
+                const int q1 = (rho - 1) / stride;
+                const int c = cmax - q1 + 1, j = stride - 1 - (rho - 1) % stride;
+                if (q1 >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
+                    double kb[4][4];
 #pragma unroll
-                for (int st = 3; st >= 0; --st) {
-                    double R[LIN_N];
-#pragma unroll
-                    for (int f = 0; f < LIN_N; ++f) R[f] = ring[(rho - 1) & 1][st][f][lane];
-                    double z[4];
-                    lin_apply<BK, CK>(R, B, C, a.grads, acc, kb[st], z);
-                    if (st > 0) {
-                        const double hs = st < 3 ? half : dt;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) kb[st - 1][q] += z[q] * hs;
+                    for (int q = 0; q < 4; ++q) {
+                        const double w = lam[q] * sixth;
+                        kb[0][q] = w; kb[3][q] = w;
+                        kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) lam[q] += z[q];
+                    for (int st = 3; st >= 0; --st) {
+                        double R[LIN_N];
+#pragma unroll
+                        for (int f = 0; f < LIN_N; ++f) R[f] = ring[(rho - 1) & 1][st][f][lane];
+                        double z[4];
+                        lin_apply<BK, CK>(R, B, C, a.grads, acc, kb[st], z);
+                        if (st > 0) {
+                            const double hs = st < 3 ? half : dt;
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) kb[st - 1][q] += z[q] * hs;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) lam[q] += z[q];
+                    }
+                    const int64_t o = ((int64_t)c * stride + j) * nr + r;
+                    if (a.g_x) lam[0] += a.g_x[o];
+                    if (a.g_y) lam[1] += a.g_y[o];
+                    if (a.g_kx) lam[2] += a.g_kx[o];
+                    if (a.g_ky) lam[3] += a.g_ky[o];
                 }
-                const int64_t o = ((int64_t)c * stride + j) * nr + r;
-                if (a.g_x) lam[0] += a.g_x[o];
-                if (a.g_y) lam[1] += a.g_y[o];
-                if (a.g_kx) lam[2] += a.g_kx[o];
-                if (a.g_ky) lam[3] += a.g_ky[o];
             }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    if (wave != chain || !active) return;
-    if (a.g_x0) a.g_x0[r] = lam[0];
-    if (a.g_y0) a.g_y0[r] = lam[1];
-    if (a.g_kx0) a.g_kx0[r] = lam[2];
-    if (a.g_ky0) a.g_ky0[r] = lam[3];
-    if (BK == BK_GRID && a.grads.depth) acc_flush(acc.h, a.grads.depth, (uint32_t)B.nx);
-    if (CK == CK_GRID) {
-        if (a.grads.u) acc_flush(acc.u, a.grads.u, (uint32_t)C.nx);
-        if (a.grads.v) acc_flush(acc.v, a.grads.v, (uint32_t)C.nx);
+        if (!active) return;
+        if (a.g_x0) a.g_x0[r] = lam[0];
+        if (a.g_y0) a.g_y0[r] = lam[1];
+        if (a.g_kx0) a.g_kx0[r] = lam[2];
+        if (a.g_ky0) a.g_ky0[r] = lam[3];
+        if (BK == BK_GRID && a.grads.depth) acc_flush(acc.h, a.grads.depth, (uint32_t)B.nx);
+        if (CK == CK_GRID) {
+            if (a.grads.u) acc_flush(acc.u, a.grads.u, (uint32_t)C.nx);
+            if (a.grads.v) acc_flush(acc.v, a.grads.v, (uint32_t)C.nx);
+        }
     }
 }
 

@@ -433,13 +433,18 @@ __device__ __forceinline__ void lin_record(const FastConsts &K, const Val64 &h, 
     double e, em;
     exp_pair(-2.0 * q, e, em, K);
     const double ope = 1.0 + e;
-    const double D = rcp_nr(em * ope);
+    // one reciprocal square root, W = 1/sqrt(g k (1-e)(1+e)), gives the
+    // reciprocal D = 1/((1-e)(1+e)) = g k W^2 and 1/sigma = W (1+e), as
+    // sigma^2 = g k tanh(kh) = g k (1-e)/(1+e)
+    const double gk = K.g * k;
+    const double W = rsq_nr((gk * em) * ope);
+    const double D = gk * (W * W);
     const double iope = em * D;
     const double T = em * iope;
     const double S = (4.0 * e) * (iope * iope);
     const double invT = (ope * ope) * D;
-    const double gkT = (K.g * k) * T;
-    const double rs = rsq_nr(gkT);
+    const double gkT = gk * T;
+    const double rs = W * ope;
     const double sigma = gkT * rs;
     const double N = __builtin_fma(q, S, T);
     const double cg = K.half_g * N * rs;
