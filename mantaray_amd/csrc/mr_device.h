@@ -277,13 +277,15 @@ __device__ __forceinline__ void wait_vmem()
     __builtin_amdgcn_s_waitcnt(0x0F70);
 }
 
-// The cell of a target, guessed in f32 and clamped into the grid (a NaN
-// guess clamps to 0): used only to choose which record to prefetch, never
-// for the result; the exact cell test happens when the record is used.
+// The cell of a target, guessed in f32 and clamped into the grid: used only
+// to choose which record to prefetch, never for the result; the exact cell
+// test happens when the record is used.  (t - a0) / sp as one FMA with the
+// loop-invariant -a0 / sp, clamped by one med3, which returns the smaller of
+// its other operands for a NaN (so a NaN guess clamps to 0).
 __device__ __forceinline__ int guess_cell(float t, float a0, float inv_sp, float max_cell)
 {
-    const float f = (t - a0) * inv_sp;
-    return (int)__builtin_fminf(__builtin_fmaxf(f, 0.0f), max_cell);
+    const float f = __builtin_fmaf(t, inv_sp, -(a0 * inv_sp));
+    return (int)__builtin_amdgcn_fmed3f(f, 0.0f, max_cell);
 }
 
 // ---------------------------------------------------------------------------
