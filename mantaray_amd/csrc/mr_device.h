@@ -33,6 +33,8 @@ namespace mr {
 
 #define MR_NANF_INIT __builtin_nanf("")
 #define MR_NAN_INIT __builtin_nan("")
+// a double whose two f32 halves are both -inf
+#define MR_EMPTY_REGION __builtin_bit_cast(double, 0xFF800000FF800000ull)
 
 constexpr double G = 9.8;                        // src/wave_ray_path.rs:23
 
@@ -654,8 +656,9 @@ __device__ __forceinline__ bool current_fast(const CurrentSlot &cs, double x, do
 // f64 pairs so that copies between slots move 64 bits per instruction.
 
 struct SharedSlot {
-    mr_d2 q[8] = {{MR_NAN_INIT, MR_NAN_INIT}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0},
-                  {0.0, 0.0},                 {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    // a slot never loaded has the empty region (four -inf bounds)
+    mr_d2 q[8] = {{MR_EMPTY_REGION, MR_EMPTY_REGION}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0},
+                  {0.0, 0.0},                         {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 };
 
 // the four f32 values of a 16-byte record piece.  The whole piece is bit-cast
@@ -673,12 +676,17 @@ __device__ __forceinline__ void shared_load(const SCell *cells, SharedSlot &s, u
     for (int k = 0; k < 8; ++k) s.q[k] = p[k];
 }
 
-// (px, py) in the slot's fast region; a never-loaded slot has NaN bounds
-// and contains no point, and so does a NaN target
+// (px, py) in the slot's fast region; a NaN target is in none
 __device__ __forceinline__ bool shared_in_region(const SharedSlot &s, float px, float py)
 {
+    // lo <= p <= hi exactly when med3(p, lo, hi) == p (med3 returns one of
+    // its operands; a NaN p fails the comparison); both axes' tests chained
+    // in vector ALU operations, one mask at the end.  An empty region is
+    // lo = hi = -inf, which only p = -inf would pass: see build_shared_kernel.
     const mr_f4 b = f4of(s.q[0]);              // (lo_x, hi_x, lo_y, hi_y)
-    return (px >= b.x) & (px <= b.y) & (py >= b.z) & (py <= b.w);
+    const float mx = __builtin_amdgcn_fmed3f(px, b.x, b.y);
+    const float my = __builtin_amdgcn_fmed3f(py, b.z, b.w);
+    return (mx == px ? my : __builtin_nanf("")) == py;
 }
 
 // Both fields at (x, y) from the slot's record, for a target in its fast
@@ -711,6 +719,20 @@ __device__ __forceinline__ void shared_fast(const SharedSlot &s, double x, doubl
 // lane evaluates from the record in hand; lanes outside its fast region
 // evaluate again through the full queries (bathy_query, current_query),
 // which find and load their cells' records and handle nodes and errors.
+// Whether any lane of the wave has c, as a wave-uniform value the compiler
+// cannot fold back into c (the empty asm).  A rare block guarded as
+//   if (__builtin_expect(wave_any(c), 0)) { __asm__ volatile(""); if (c) ... }
+// stays behind a scalar branch: when no lane has c, the wave passes it
+// without writing EXEC, and the block is laid out after the loop, so the
+// common path falls through (the volatile asm keeps the compiler from
+// merging the two tests into one EXEC-masked branch over an inline block).
+__device__ __forceinline__ bool wave_any(bool c)
+{
+    uint64_t m = __builtin_amdgcn_ballot_w64(c);
+    __asm__("" : "+s"(m));
+    return m != 0;
+}
+
 template <int MATH>
 __device__ __forceinline__ void rhs_shared(const BathyView &B, const BathyPtrs &BP,
                                            const CurrentView &C, const CurrentPtrs &CP,
@@ -722,12 +744,15 @@ __device__ __forceinline__ void rhs_shared(const BathyView &B, const BathyPtrs &
     bool slow;
     shared_fast(ss, x, y, h, hx, hy, u, v, ux, uy, vx, vy, slow);
     odes<MATH>(K, h, hx, hy, true, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
-    if (slow) {
-        BathySlot bs;
-        CurrentSlot cs;
-        const double h2 = bathy_query<BK_GRID>(B, BP, bs, x, y, hx, hy);
-        const bool okc = current_query<CK_GRID>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
-        odes<MATH>(K, h2, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+    if (__builtin_expect(wave_any(slow), 0)) {
+        __asm__ volatile("");
+        if (slow) {
+            BathySlot bs;
+            CurrentSlot cs;
+            const double h2 = bathy_query<BK_GRID>(B, BP, bs, x, y, hx, hy);
+            const bool okc = current_query<CK_GRID>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
+            odes<MATH>(K, h2, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+        }
     }
 }
 
@@ -752,10 +777,14 @@ __device__ __forceinline__ void rhs(const BathyView &B, const BathyPtrs &BP, Bat
         const bool okc = CK == CK_GRID ? current_fast(cs, x, y, u, v, ux, uy, vx, vy, sc)
                                        : current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
         odes<MATH>(K, h, hx, hy, okc, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
-        if (sb | sc) {
-            const double h2 = bathy_query<BK>(B, BP, bs, x, y, hx, hy);
-            const bool okc2 = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
-            odes<MATH>(K, h2, hx, hy, okc2, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+        const bool slow = sb | sc;
+        if (__builtin_expect(wave_any(slow), 0)) {
+            __asm__ volatile("");
+            if (slow) {
+                const double h2 = bathy_query<BK>(B, BP, bs, x, y, hx, hy);
+                const bool okc2 = current_query<CK>(C, CP, cs, x, y, u, v, ux, uy, vx, vy);
+                odes<MATH>(K, h2, hx, hy, okc2, u, v, ux, uy, vx, vy, kx, ky, dx, dy, dkx, dky);
+            }
         }
         return;
     }

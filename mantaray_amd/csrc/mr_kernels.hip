@@ -78,8 +78,9 @@ __device__ __forceinline__ void rk4_step(const BathyView &B, const BathyPtrs &BP
         if (st == 0) {
             // With MR_SOLOUT_NEXT_DERIV, solout saw f(y) of the row
             // just pushed: an all-NaN k1 here means the ray already
-            // ended on that row.
-            if (solout == 1) {
+            // ended on that row.  (Marked unlikely: the test is laid out
+            // after the loop, and the default reading falls through.)
+            if (__builtin_expect(solout == 1, 0)) {
                 stop_before = all_nan4(dx, dy, dkx, dky);
                 if (stop_before) on_stop();
             }
@@ -156,7 +157,8 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
                          dy, dkx, dky);
         px = dx; py = dy;
         if (st == 0) {
-            if (solout == 1) {
+            // the MR_SOLOUT_NEXT_DERIV test, as in rk4_step
+            if (__builtin_expect(solout == 1, 0)) {
                 stop_before = all_nan4(dx, dy, dkx, dky);
                 if (stop_before) on_stop();
             }
@@ -630,12 +632,17 @@ __global__ void build_shared_kernel(SharedBuildArgs a)
         r.vx = (a.v[iD] - a.v[iA]) / a.cxspace;
         r.vy = (a.v[iB] - a.v[iA]) / a.cyspace;
     }
+    // An empty region is lo = hi = -inf: the region test (shared_in_region,
+    // med3(p, lo, hi) == p) passes no finite target and no NaN.  p = -inf
+    // passes it; such a lane evaluates this record's NaN pieces (every piece
+    // NaN past the data; m01 NaN where det == 0), which give NaN h, u and v
+    // and so all-NaN derivatives, as the reference's error does.
     const float2 bx = a.fx[i], by = a.fy[j];
-    const float inf = __builtin_inff();
-    r.lo_x = err ? inf : bx.x;
-    r.hi_x = err ? -inf : bx.y;
-    r.lo_y = err ? inf : by.x;
-    r.hi_y = err ? -inf : by.y;
+    const float ninf = -__builtin_inff();
+    r.lo_x = err ? ninf : bx.x;
+    r.hi_x = err ? ninf : bx.y;
+    r.lo_y = err ? ninf : by.x;
+    r.hi_y = err ? ninf : by.y;
     a.cells[tiled_cell((int)i, (int)j, a.ntx)] = r;
 }
 
