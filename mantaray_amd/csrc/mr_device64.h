@@ -336,11 +336,14 @@ __device__ __forceinline__ void rhs64(const Grid64View &B, const Grid64Ptrs &BP,
     }
     odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy, v.gx, v.gy,
                kx, ky, dx, dy, dkx, dky);
-    if (slow) {                        // a NaN target lands here too
-        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
-        odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy, v.gx,
-                   v.gy, kx, ky, dx, dy, dkx, dky);
+    if (__builtin_expect(wave_any(slow), 0)) {
+        __asm__ volatile("");
+        if (slow) {                    // a NaN target lands here too
+            okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+            okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+            odes<MATH>(K, okb ? h.val : MR_NAN, h.gx, h.gy, okc, u.val, v.val, u.gx, u.gy,
+                       v.gx, v.gy, kx, ky, dx, dy, dkx, dky);
+        }
     }
 }
 
@@ -372,9 +375,13 @@ __device__ __forceinline__ void acc_add(CornerAcc &s, double *grad, uint32_t nx,
                                         const Cell64 &cell, double rxs, double rys, double vbar,
                                         double gxbar, double gybar)
 {
-    if (cell.iA != s.iA) {
-        acc_flush(s, grad, nx);
-        s.iA = cell.iA;
+    const bool moved = cell.iA != s.iA;
+    if (__builtin_expect(wave_any(moved), 0)) {
+        __asm__ volatile("");
+        if (moved) {
+            acc_flush(s, grad, nx);
+            s.iA = cell.iA;
+        }
     }
     const double u = cell.u, w = cell.w;
     const double gx = gxbar * rxs, gy = gybar * rys;
@@ -505,10 +512,13 @@ __device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs 
         okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     }
     lin_record<BK, CK>(K, h, u, v, bc, cc, okb, okc, kx, ky, R);
-    if (slow) {
-        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
-        lin_record<BK, CK>(K, h, u, v, bc, cc, okb, okc, kx, ky, R);
+    if (__builtin_expect(wave_any(slow), 0)) {
+        __asm__ volatile("");
+        if (slow) {
+            okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+            okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+            lin_record<BK, CK>(K, h, u, v, bc, cc, okb, okc, kx, ky, R);
+        }
     }
 }
 
@@ -577,9 +587,12 @@ __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs 
     } else {
         okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     }
-    if (slow) {
-        okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
-        okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+    if (__builtin_expect(wave_any(slow), 0)) {
+        __asm__ volatile("");
+        if (slow) {
+            okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
+            okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
+        }
     }
     const double hh = h.val;
     const double k2 = __builtin_fma(kx, kx, ky * ky);
