@@ -679,14 +679,16 @@ __device__ __forceinline__ void shared_load(const SCell *cells, SharedSlot &s, u
 // (px, py) in the slot's fast region; a NaN target is in none
 __device__ __forceinline__ bool shared_in_region(const SharedSlot &s, float px, float py)
 {
-    // lo <= p <= hi exactly when med3(p, lo, hi) == p (med3 returns one of
-    // its operands; a NaN p fails the comparison); both axes' tests chained
-    // in vector ALU operations, one mask at the end.  An empty region is
-    // lo = hi = -inf, which only p = -inf would pass: see build_shared_kernel.
+    // lo <= p <= hi exactly when med3(p, lo, hi) - p is zero (med3 returns
+    // one of its operands, and with f32 denormals kept a difference of two
+    // finite floats is zero only when they are equal); the two axes'
+    // differences are tested at once, in vector ALU operations with one mask
+    // at the end.  A NaN or infinite p makes a difference NaN or infinite, so
+    // it is in no region; the empty region, lo = hi = -inf, holds no target.
     const mr_f4 b = f4of(s.q[0]);              // (lo_x, hi_x, lo_y, hi_y)
-    const float mx = __builtin_amdgcn_fmed3f(px, b.x, b.y);
-    const float my = __builtin_amdgcn_fmed3f(py, b.z, b.w);
-    return (mx == px ? my : __builtin_nanf("")) == py;
+    const float dx = __builtin_amdgcn_fmed3f(px, b.x, b.y) - px;
+    const float dy = __builtin_amdgcn_fmed3f(py, b.z, b.w) - py;
+    return __builtin_fabsf(dx) + __builtin_fabsf(dy) == 0.0f;
 }
 
 // Both fields at (x, y) from the slot's record, for a target in its fast

@@ -146,11 +146,16 @@ __device__ __forceinline__ void rk4_step_shared(const BathyView &B, const BathyP
             const float g = st == 1 ? g1 : g3;
             const float gx = __builtin_fmaf((float)px, g, (float)x);
             const float gy = __builtin_fmaf((float)py, g, (float)y);
-            if (!shared_in_region(refill, gx, gy)) {
-                const int i1 = guess_cell(gx, B.ax0, B.pinv_sx, B.pmax_x);
-                const int j1 = guess_cell(gy, B.ay0, B.pinv_sy, B.pmax_y);
-                shared_load(cells, refill, tiled_cell(i1, j1, B.ntx));
-            }
+            // The record's offset is formed before the test, in every step:
+            // some lane of the wave refills in most steps anyway, and these
+            // vector operations then stand between the test's compare and
+            // the scalar EXEC update that reads it, which otherwise waits for
+            // the compare's result (the empty asm keeps them out of the
+            // conditional block).
+            uint32_t cell = tiled_cell(guess_cell(gx, B.ax0, B.pinv_sx, B.pmax_x),
+                                       guess_cell(gy, B.ay0, B.pinv_sy, B.pmax_y), B.ntx);
+            __asm__ volatile("" : "+v"(cell));
+            if (!shared_in_region(refill, gx, gy)) shared_load(cells, refill, cell);
         }
         double dx, dy, dkx, dky;
         rhs_shared<MATH>(B, BP, C, CP, (st == 0 || st == 3) ? sa : sb, K, sx, sy, skx, sky, dx,
@@ -632,11 +637,9 @@ __global__ void build_shared_kernel(SharedBuildArgs a)
         r.vx = (a.v[iD] - a.v[iA]) / a.cxspace;
         r.vy = (a.v[iB] - a.v[iA]) / a.cyspace;
     }
-    // An empty region is lo = hi = -inf: the region test (shared_in_region,
-    // med3(p, lo, hi) == p) passes no finite target and no NaN.  p = -inf
-    // passes it; such a lane evaluates this record's NaN pieces (every piece
-    // NaN past the data; m01 NaN where det == 0), which give NaN h, u and v
-    // and so all-NaN derivatives, as the reference's error does.
+    // An empty region is lo = hi = -inf, which holds no target under the
+    // region test (shared_in_region): a target in a cell where the reference
+    // errs always takes the full queries.
     const float2 bx = a.fx[i], by = a.fy[j];
     const float ninf = -__builtin_inff();
     r.lo_x = err ? ninf : bx.x;

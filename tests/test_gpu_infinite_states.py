@@ -1,13 +1,14 @@
 """Rays that start at infinite positions, on grids with error cells.
 
-The shared-cell trace tests a target against a record's fast region as
-med3(p, lo, hi) == p per axis, and a cell where the reference errs (a corner
-past a value array) has the empty region lo = hi = -inf.  A finite or NaN
-target is in no empty region; p = -inf is, and then evaluates the error
-cell's NaN pieces, which must give the reference's all-NaN row.  A ray at
-x = -inf guesses cell column 0, so with y in the last cell rows of a grid
-whose value arrays are cut short (every cell of those rows errs) it meets
-exactly that case; x = +inf and y = +-inf take the full queries instead.
+The shared-cell trace tests a target against a record's fast region
+through med3(p, lo, hi) - p per axis, and a cell where the reference errs (a
+corner past a value array) has the empty region lo = hi = -inf.  An
+infinite target makes a difference infinite or NaN and must take the full
+queries, which err, giving the reference's all-NaN row.  A ray at x = -inf
+guesses cell column 0 and one at x = +inf the last, so with y in the last
+cell rows of a grid whose value arrays are cut short (every cell of those
+rows errs) the record in hand has the empty region, whose bounds equal that
+target; x = +inf and y = +-inf meet ordinary records too.
 Row counts, NaN patterns and values against the oracle, for both math modes,
 with trajectories (buffer-store kernel) and endpoints only.
 """
