@@ -717,10 +717,6 @@ __device__ __forceinline__ void shared_fast(const SharedSlot &s, double x, doubl
     ux = s.q[6].x; uy = s.q[6].y; vx = s.q[7].x; vy = s.q[7].y;
 }
 
-// WaveRayPath::system for gridded depth and current on shared cells: every
-// lane evaluates from the record in hand; lanes outside its fast region
-// evaluate again through the full queries (bathy_query, current_query),
-// which find and load their cells' records and handle nodes and errors.
 // Whether any lane of the wave has c, as a wave-uniform value the compiler
 // cannot fold back into c (the empty asm).  A rare block guarded as
 //   if (__builtin_expect(wave_any(c), 0)) { __asm__ volatile(""); if (c) ... }
@@ -735,6 +731,10 @@ __device__ __forceinline__ bool wave_any(bool c)
     return m != 0;
 }
 
+// WaveRayPath::system for gridded depth and current on shared cells: every
+// lane evaluates from the record in hand; lanes outside its fast region
+// evaluate again through the full queries (bathy_query, current_query),
+// which find and load their cells' records and handle nodes and errors.
 template <int MATH>
 __device__ __forceinline__ void rhs_shared(const BathyView &B, const BathyPtrs &BP,
                                            const CurrentView &C, const CurrentPtrs &CP,
