@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -898,7 +899,7 @@ int adjoint_run(const mr_field *bathy, const mr_field *current, const double *de
     if ((g_u || g_v) && current->kind != MR_CURRENT_GRID)
         return fail(MR_ERR_INVALID_ARGUMENT, "current gradient requested of a non-gridded current");
     int64_t need = 0;
-    mr_adjoint_workspace_size(n_rays, stride, &need);
+    if ((rc = mr_adjoint_workspace_size(n_rays, stride, &need))) return rc;
     if (no_alloc && need > MR_ADJOINT_SCRATCH_BYTES)
         return fail(MR_ERR_INVALID_ARGUMENT,
                     "the sweep needs " + std::to_string(need) + " bytes of work buffer (" +
@@ -1487,7 +1488,10 @@ int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes)
     if (n_rays < 0 || stride <= 0) return fail(MR_ERR_INVALID_ARGUMENT, "invalid sizes");
     // two halves (the recompute and the reverse sweep alternate between
     // them), each stride steps x 16 doubles x n_rays
-    *bytes = 2 * (int64_t)stride * 16 * n_rays * (int64_t)sizeof(double);
+    const int64_t per_ray = 2 * (int64_t)stride * 16 * (int64_t)sizeof(double);
+    if (n_rays > INT64_MAX / per_ray)
+        return fail(MR_ERR_INVALID_ARGUMENT, "work buffer size does not fit in 64 bits");
+    *bytes = per_ray * n_rays;
     return MR_OK;
     MR_GUARD_END
 }

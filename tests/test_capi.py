@@ -192,3 +192,28 @@ def test_adjoint_noalloc_refuses_oversized_work_on_the_host():
     assert "holds" not in L.mr_last_error().decode() or rc == 0
     L.mr_field_free(b)
     L.mr_field_free(c)
+
+
+def test_workspace_size_overflow_is_an_error():
+    """A work buffer whose size does not fit in 64 bits is an argument error,
+    from the size query and from the sweep itself, before any device work."""
+    from mantaray_amd import _capi as C
+    L = C.lib()
+    need = ctypes.c_int64(-1)
+    assert L.mr_adjoint_workspace_size(1 << 40, (1 << 31) - 1, ctypes.byref(need)) == \
+        C.MR_ERR_INVALID_ARGUMENT
+    assert "64 bits" in L.mr_last_error().decode()
+    assert need.value == -1
+    C.check(L.mr_adjoint_workspace_size(1 << 20, 50, ctypes.byref(need)))
+    assert need.value == 2 * 50 * 16 * 8 << 20
+    b, c = ctypes.c_void_p(), ctypes.c_void_p()
+    C.check(L.mr_bathy_constant(30.0, ctypes.byref(b)))
+    C.check(L.mr_current_constant(0.0, 0.0, ctypes.byref(c)))
+    o = C.options(memory=C.MR_MEM_DEVICE)
+    rc = L.mr_adjoint_many(b, c, None, None, None, 1 << 40, 0.0, 100.0, 1.0, (1 << 31) - 1,
+                           ctypes.byref(o), 1, 1, 1, None, None, None, None, None, 1, 1 << 62,
+                           None, None, None, None, None, None, None)
+    assert rc == C.MR_ERR_INVALID_ARGUMENT
+    assert "64 bits" in L.mr_last_error().decode()
+    L.mr_field_free(b)
+    L.mr_field_free(c)
