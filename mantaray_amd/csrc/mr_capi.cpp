@@ -710,6 +710,8 @@ int trace64_device(const mr_field *bathy, const mr_field *current, const double 
     a.stride = stride;
     a.ckpt = ckpt;
     const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    if ((n_rays + block - 1) / block >= (int64_t)INT32_MAX)
+        return fail(MR_ERR_UNSUPPORTED, "too many rays for one launch");
     a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace64(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
                               stream));
@@ -868,6 +870,8 @@ int trace_device(const mr_field *bathy, const mr_field *current, int64_t n_rays,
     if (a.shared_cells && (rc = shared_tables(bathy, current, o.record_tile, &a.scells)))
         return rc;
     const int block = o.block_size ? o.block_size : tuned_block(n_rays);
+    if ((n_rays + block - 1) / block >= (int64_t)INT32_MAX)
+        return fail(MR_ERR_UNSUPPORTED, "too many rays for one launch");
     a.ray_blocks = (int)((n_rays + block - 1) / block);
     MR_HIP_TRY(launch_trace(a, bathy_kind_of(bathy), current_kind_of(current), o.math, block,
                             stream));

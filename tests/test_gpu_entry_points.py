@@ -146,3 +146,29 @@ def test_benched_launch_against_the_oracle(oracle_mod, seamount, record_tile):
     dk = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("kx", "ky")) / kscale
     print(f"record tile {record_tile}: max |d(x, y)| {dxy:.3e} m, relative |dk| {dk:.3e}")
     assert dxy <= 1e-6 and dk <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["reference", "fp64"])
+def test_too_many_rays_for_one_launch(precision):
+    """A ray count whose grid of workgroups would not fit a launch is refused
+    with MR_ERR_UNSUPPORTED before anything is launched (the pointers are
+    never read: a few real device words stand in for them)."""
+    import torch
+
+    from mantaray_amd import _capi as C
+    L = C.lib()
+    b, c = ctypes.c_void_p(), ctypes.c_void_p()
+    C.check(L.mr_bathy_constant(30.0, ctypes.byref(b)))
+    C.check(L.mr_current_constant(0.0, 0.0, ctypes.byref(c)))
+    dummy = torch.zeros(8, dtype=torch.float64, device="cuda:0")
+    p = dummy.data_ptr()
+    lenbuf = torch.zeros(8, dtype=torch.int32, device="cuda:0").data_ptr()
+    prec = C.MR_PREC_REFERENCE if precision == "reference" else C.MR_PREC_FP64_INTERP
+    o = C.options(memory=C.MR_MEM_DEVICE, device=0, precision=prec, block_size=64)
+    rc = L.mr_trace_many(b, c, 1 << 40, p, p, p, p, 0.0, 2.0, 1.0, ctypes.byref(o),
+                         None, None, None, None, None, lenbuf, None)
+    assert rc == C.MR_ERR_UNSUPPORTED
+    assert "too many rays" in L.mr_last_error().decode()
+    torch.cuda.synchronize()
+    L.mr_field_free(b)
+    L.mr_field_free(c)
