@@ -741,7 +741,12 @@ bool shared_cells(const mr_field *b, const mr_field *c)
 // current's f64 index rule (T64[i] <= x < T64[i+1]), and are strictly
 // between the cell's node coordinates a[i] and a[i+1] (no corner).  If
 // RN32(x) >= lo > ceil32(T64[i]) then x > T64[i]; if RN32(x) <= hi <
-// floor32(T64[i+1]) then x < T64[i+1] (rounding is monotone).
+// floor32(T64[i+1]) then x < T64[i+1] (rounding is monotone).  The three
+// intervals need not meet (on axes whose spacing varies, the index rules,
+// which use the first spacing, can map a cell's targets outside its nodes):
+// an empty interval is stored as (-inf, -inf), the one form of an empty
+// region that the device's region test (med3(p, lo, hi) - p) reads as
+// empty; lo > hi would read as [hi, lo].
 std::vector<float2> fast_bounds(const std::vector<float> &a32, const std::vector<double> &a64)
 {
     const std::vector<float> t32 = thresholds32(a32);
@@ -760,7 +765,7 @@ std::vector<float2> fast_bounds(const std::vector<float> &a32, const std::vector
     for (size_t i = 0; i + 1 < a32.size(); i++) {
         const float lo = std::max({t32[i], up(a32[i]), up(ceil32(t64[i]))});
         const float hi = std::min({down(t32[i + 1]), down(a32[i + 1]), down(floor32(t64[i + 1]))});
-        b[i] = make_float2(lo, hi);
+        b[i] = lo <= hi ? make_float2(lo, hi) : make_float2(-INFINITY, -INFINITY);
     }
     return b;
 }

@@ -1,0 +1,98 @@
+"""Randomized traces against the oracle.
+
+Twelve small scenes drawn from a seed: grid sizes 6-40 nodes a side, random
+spacing and origin, axes uniform or jittered by up to a fifth of a cell (the
+reference's index rule uses the first spacing, (p - a0)/|a1 - a0|, and the
+interpolation the true corner coordinates; oracle and product both follow
+it), rough depth that crosses zero in places (h <= 0 ends a ray), rough
+current, and a step of 0.3, 0.5, 0.75, 1 or 1.7 s.  Depth and current share
+their cells in most scenes (the shared-cell kernel, with dt a power of two
+or not) and are offset by a fraction of a cell in the rest (separate
+records).  The states include exact nodes, cell lines, the last row and
+column, points off the grid, NaN and zero k.  Each scene runs in both math
+modes and both solout readings, with trajectories and endpoints only:
+identical row counts and NaN patterns, positions within 1e-6 m,
+wavenumbers within 1e-8 of their scale, and endpoints equal to the last
+rows.
+"""
+import numpy as np
+import pytest
+
+from _helpers import oracle_fields, random_states
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mr():
+    import mantaray_amd
+    from mantaray_amd import _capi
+    _capi.lib()
+    return mantaray_amd
+
+
+def _scene(seed):
+    rng = np.random.default_rng(7000 + seed)
+    nx, ny = (int(v) for v in rng.integers(6, 41, size=2))
+    dx, dy = rng.uniform(20.0, 80.0, size=2)
+    x0, y0 = rng.uniform(-1000.0, 1000.0, size=2)
+    x = x0 + dx * np.arange(nx)
+    y = y0 + dy * np.arange(ny)
+    if seed % 2:
+        x[1:] += rng.uniform(-0.2, 0.2, nx - 1) * dx
+        y[1:] += rng.uniform(-0.2, 0.2, ny - 1) * dy
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    X, Y = np.meshgrid(x32.astype(np.float64), y32.astype(np.float64))
+    depth = (rng.uniform(5.0, 40.0) + rng.uniform(5.0, 30.0)
+             * np.sin(X / rng.uniform(60.0, 300.0) + rng.uniform(0, 6))
+             * np.cos(Y / rng.uniform(60.0, 300.0)) + rng.normal(0.0, 2.0, X.shape))
+    u = rng.uniform(-0.8, 0.8) * np.sin(Y / rng.uniform(80.0, 300.0)) + rng.normal(0, 0.05, X.shape)
+    v = rng.uniform(-0.8, 0.8) * np.cos(X / rng.uniform(80.0, 300.0)) + rng.normal(0, 0.05, X.shape)
+    cx, cy = x32.astype(np.float64), y32.astype(np.float64)
+    if seed % 4 == 3:                                    # separate cells
+        cx = cx + 0.31 * dx
+        cy = cy - 0.17 * dy
+    cfg = {"bathy": {"x": x32, "y": y32, "depth": depth},
+           "current": {"x": cx, "y": cy, "u": u, "v": v}}
+    dt = float(rng.choice([0.3, 0.5, 0.75, 1.0, 1.7]))
+    steps = int(rng.integers(20, 61))
+    return cfg, dt, steps
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_scene(mr, oracle_mod, seed):
+    import torch
+
+    cfg, dt, steps = _scene(seed)
+    b = mr.Bathymetry.from_arrays(**cfg["bathy"])
+    c = mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 512)
+    t_end = steps * dt
+    ended = 0
+    for math in ("fast", "literal"):
+        for solout in ("state", "next_deriv"):
+            osol = oracle_mod.SOLOUT_STATE if solout == "state" else oracle_mod.SOLOUT_NEXT_DERIV
+            ref = oracle_mod.trace_many(ob, oc, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, t_end,
+                                        dt, solout=osol)
+            got = mr.trace_many(b, c, s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, t_end, dt,
+                                math=math, solout=solout)
+            tag = (seed, math, solout)
+            assert np.array_equal(got["len"], ref["len"]), tag
+            for k in ("x", "y", "kx", "ky"):
+                assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), (tag, k)
+            dxy = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("x", "y"))
+            kscale = np.nanmax(np.abs(ref["kx"])) + np.nanmax(np.abs(ref["ky"]))
+            dk = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("kx", "ky")) / kscale
+            assert dxy <= 1e-6 and dk <= 1e-8, (tag, dxy, dk)
+            ic = [torch.tensor(np.ascontiguousarray(s[:, q]), device="cuda:0") for q in range(4)]
+            dev = mr.trace_many_device(b, c, *ic, 0.0, t_end, dt, math=math, solout=solout,
+                                       trajectories=False)
+            assert np.array_equal(dev["len"].cpu().numpy(), got["len"]), tag
+            end = dev["end"].cpu().numpy()
+            assert np.array_equal(np.isnan(end), np.isnan(got["end"])), tag
+            fin = ~np.isnan(end)
+            assert np.array_equal(end[fin], got["end"][fin]), tag
+            ended = max(ended, int((ref["len"] < steps + 1).sum()))
+    print(f"seed {seed}: {cfg['bathy']['x'].size} x {cfg['bathy']['y'].size} nodes, dt {dt}, "
+          f"{steps} steps, {ended} of 512 rays end early")
