@@ -13,7 +13,8 @@ column, points off the grid, NaN and zero k.  Each scene runs in both math
 modes and both solout readings, with trajectories and endpoints only:
 identical row counts and NaN patterns, positions within 1e-6 m,
 wavenumbers within 1e-8 of their scale, and endpoints equal to the last
-rows.
+rows.  The same scenes also run together in one batched launch, and in the
+fp64-interpolation mode the adjoint differentiates.
 """
 import numpy as np
 import pytest
@@ -96,3 +97,53 @@ def test_random_scene(mr, oracle_mod, seed):
             ended = max(ended, int((ref["len"] < steps + 1).sum()))
     print(f"seed {seed}: {cfg['bathy']['x'].size} x {cfg['bathy']['y'].size} nodes, dt {dt}, "
           f"{steps} steps, {ended} of 512 rays end early")
+
+
+def test_random_scenes_batched(mr):
+    """All twelve scenes in one persistent launch: every ray's row count and
+    last row equal bitwise those of its scene's own trace (some scenes share
+    cells and some do not, so the batch runs on separate records while most
+    single traces use the shared-cell records)."""
+    import torch
+    from mantaray_amd.batch import Batch
+    scenes = []
+    for seed in range(12):
+        cfg, dt, steps = _scene(seed)
+        s = random_states(seed, cfg, 512)
+        scenes.append({"bathymetry": mr.Bathymetry.from_arrays(**cfg["bathy"]),
+                       "current": mr.Current.from_arrays(**cfg["current"]),
+                       "x0": s[:, 0], "y0": s[:, 1], "kx0": s[:, 2], "ky0": s[:, 3],
+                       "t0": 0.0, "t_end": steps * dt, "dt": dt})
+    batch = Batch(scenes)
+    batch.run()
+    torch.cuda.synchronize()
+    for i, (sc, out) in enumerate(zip(scenes, batch.results())):
+        ref = mr.trace_many(sc["bathymetry"], sc["current"], sc["x0"], sc["y0"], sc["kx0"],
+                            sc["ky0"], sc["t0"], sc["t_end"], sc["dt"], trajectories=False)
+        assert np.array_equal(out["len"].cpu().numpy(), ref["len"]), i
+        end = out["end"].cpu().numpy()
+        assert np.array_equal(np.isnan(end), np.isnan(ref["end"])), i
+        fin = ~np.isnan(end)
+        assert np.array_equal(end[fin], ref["end"][fin]), i
+    batch.close()
+
+
+@pytest.mark.parametrize("seed", range(0, 12, 3))
+def test_random_scene_fp64_interpolation(mr, oracle_mod, seed):
+    """The fp64-interpolation trace (the path the adjoint differentiates) on
+    the same scenes against the oracle in that mode: identical row counts and
+    NaN patterns, positions within 1e-7 m."""
+    cfg, dt, steps = _scene(seed)
+    b = mr.Bathymetry.from_arrays(**cfg["bathy"])
+    c = mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 512)
+    args = (s[:, 0], s[:, 1], s[:, 2], s[:, 3], 0.0, steps * dt, dt)
+    got = mr.trace_many(b, c, *args, precision="fp64")
+    ref = oracle_mod.trace_many(ob, oc, *args, prec=oracle_mod.PREC_FP64_INTERP)
+    assert np.array_equal(got["len"], ref["len"])
+    for k in ("x", "y", "kx", "ky"):
+        assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), k
+    d = max(np.nanmax(np.abs(got[k] - ref[k])) for k in ("x", "y"))
+    print(f"seed {seed} fp64: max |d(x, y)| {d:.2e} m")
+    assert d <= 1e-7
