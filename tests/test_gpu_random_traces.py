@@ -200,3 +200,22 @@ def test_random_analytic_sources(mr, oracle_mod, kind, current):
             assert np.array_equal(np.isnan(got[key]), np.isnan(ref[key])), (math, key)
         d = max(np.nanmax(np.abs(got[key] - ref[key])) for key in ("x", "y"))
         assert d <= 1e-6, (math, d)
+
+
+@pytest.mark.parametrize("seed", [1, 3, 5, 7])
+@pytest.mark.parametrize("math", ["fast", "literal"])
+def test_random_scene_rhs(mr, oracle_mod, seed, math):
+    """The right-hand side (the full field queries, no cell records carried
+    between evaluations) at the scenes' special states on jittered axes,
+    against the oracle: identical NaN patterns, values within the tolerance
+    of the uniform-grid RHS tests."""
+    cfg, dt, steps = _scene(seed)
+    b = mr.Bathymetry.from_arrays(**cfg["bathy"])
+    c = mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    s = random_states(seed, cfg, 4096)
+    ds = mr.rhs(b, c, s, math=math)
+    ref, st = oracle_mod.system_many(ob, oc, s)
+    assert np.array_equal(np.isnan(ds), np.isnan(ref))
+    scale = np.nanmax(np.abs(ref), axis=0)
+    assert np.nanmax(np.abs(ds - ref) / scale) <= (1e-12 if math == "literal" else 1e-11)
