@@ -219,3 +219,82 @@ def test_random_scene_rhs(mr, oracle_mod, seed, math):
     assert np.array_equal(np.isnan(ds), np.isnan(ref))
     scale = np.nanmax(np.abs(ref), axis=0)
     assert np.nanmax(np.abs(ds - ref) / scale) <= (1e-12 if math == "literal" else 1e-11)
+
+
+@pytest.mark.parametrize("seed", [1, 3])
+def test_random_scene_gradients(mr, oracle_mod, seed):
+    """Gradients on jittered axes (shared cells for seed 1, separate for 3):
+    the 4 x 4 Jacobian of each of 64 rays' end state with respect to its
+    initial state, by four reverse sweeps with unit cotangents, and the
+    derivative of sum_r x_r(T) along a random direction of u, against
+    central differences of the long-double trace (eps 1e-6 m for positions,
+    1e-12 for wavenumbers, 1e-6 for u).  Rays whose two perturbed traces met
+    different cells (where the cell slopes jump) or that end on an all-NaN
+    row (which passes no gradient) are left out; per ray and
+    end component the Jacobian row agrees within 1e-6 of its largest entry,
+    and the directional derivative within 1e-6 of its terms."""
+    import torch
+
+    from mantaray_amd import adjoint
+    cfg, dt, steps = _scene(seed)
+    b = mr.Bathymetry.from_arrays(**cfg["bathy"])
+    c = mr.Current.from_arrays(**cfg["current"])
+    ob, oc = oracle_fields(oracle_mod, cfg)
+    rng = np.random.default_rng(500 + seed)
+    x, y = cfg["bathy"]["x"].astype(np.float64), cfg["bathy"]["y"].astype(np.float64)
+    n = 64
+    th = rng.uniform(0.0, 2.0 * np.pi, n)
+    k = rng.uniform(0.03, 0.2, n)
+    s0 = np.stack([rng.uniform(x[1], x[-2], n), rng.uniform(y[1], y[-2], n),
+                   k * np.cos(th), k * np.sin(th)], 1)
+    T = steps * dt
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda:0")  # noqa: E731
+    ic = [dev(s0[:, q]) for q in range(4)]
+    u, v = dev(cfg["current"]["u"].ravel()), dev(cfg["current"]["v"].ravel())
+    stride = 10
+    fwd = adjoint.trace_fp64(b, c, *ic, 0.0, T, dt, stride=stride, u=u, v=v)
+    J = np.zeros((n, 4, 4))
+    gu = None
+    for q in range(4):
+        ge = torch.zeros((n, 4), dtype=torch.float64, device="cuda:0")
+        ge[:, q] = 1.0
+        g = adjoint.adjoint(b, c, fwd, 0.0, T, dt, stride, g_end=ge, u=u, v=v,
+                            want=("x0", "y0", "kx0", "ky0", "u"))
+        for j, name in enumerate(("x0", "y0", "kx0", "ky0")):
+            J[:, q, j] = g[name].cpu().numpy()
+        if q == 0:
+            gu = g["u"].cpu().numpy()
+    scale = np.array([1e-6, 1e-6, 1e-12, 1e-12])
+    Jfd = np.zeros_like(J)
+    same = np.ones(n, dtype=bool)
+    for j in range(4):
+        ds0 = np.zeros_like(s0)
+        ds0[:, j] = 1.0
+        diff, _, lp, lm, sc = oracle_mod.fd_end_ld(ob, oc, s0, 0.0, T, dt, eps=scale[j],
+                                                   ds0=ds0, with_cells=True)
+        Jfd[:, :, j] = diff / (2 * scale[j])
+        # both perturbed traces in the same cells, and both with the full
+        # row count and a finite last row (a ray whose last row is all-NaN
+        # passes no gradient through it)
+        same &= sc & (lp == lm) & (lp == steps + 1) & np.isfinite(diff).all(axis=1)
+    assert same.sum() >= 16, int(same.sum())
+    A, B = J * scale, Jfd * scale
+    err = np.abs(A - B).max(axis=2) / np.abs(B).max(axis=2)
+    print(f"seed {seed}: {int(same.sum())} of {n} rays usable; per-ray Jacobian rows "
+          f"max error {err[same].max():.2e}")
+    assert err[same].max() <= 1e-6
+    # d(sum of x_r(T) over the usable rays)/du along a random direction
+    d = rng.normal(0.0, 1.0, cfg["current"]["u"].size)
+    diff, _, lp, lm, sc = oracle_mod.fd_end_ld(ob, oc, s0[same], 0.0, T, dt, eps=1e-6, du=d,
+                                               with_cells=True)
+    ok = sc & (lp == lm)
+    fd = float(np.sum(diff[ok, 0])) / 2e-6
+    ge = torch.zeros((n, 4), dtype=torch.float64, device="cuda:0")
+    idx = np.nonzero(same)[0][ok]
+    ge[torch.tensor(idx, device="cuda:0"), 0] = 1.0
+    g = adjoint.adjoint(b, c, fwd, 0.0, T, dt, stride, g_end=ge, u=u, v=v, want=("u",))
+    ad = float(np.dot(g["u"].cpu().numpy(), d))
+    terms = float(np.sum(np.abs(g["u"].cpu().numpy() * d)))
+    print(f"seed {seed}: d(sum x)/du along a random direction: adjoint {ad:.9e}, "
+          f"long-double FD {fd:.9e} over {int(ok.sum())} rays")
+    assert abs(ad - fd) <= 1e-6 * terms
