@@ -217,3 +217,27 @@ def test_workspace_size_overflow_is_an_error():
     assert "64 bits" in L.mr_last_error().decode()
     L.mr_field_free(b)
     L.mr_field_free(c)
+
+
+def test_num_steps_property(oracle_mod):
+    """mr_num_steps against the oracle's restatement of Rk4::integrate's step
+    count over arbitrary doubles (NaN, infinities, subnormals, extreme
+    ratios): the same count wherever the reference has one and it is below
+    2^31 - 1 (the product's limit), MR_ERR_STEP_COUNT everywhere else."""
+    from hypothesis import given, settings, strategies as st
+
+    fl = st.floats(allow_nan=True, allow_infinity=True, width=64)
+    small = st.floats(min_value=-1e4, max_value=1e4, allow_nan=False)
+
+    @settings(max_examples=3000, deadline=None, derandomize=True)
+    @given(st.one_of(fl, small), st.one_of(fl, small), st.one_of(fl, small,
+                                                               st.sampled_from([0.1, 0.3, 1.7])))
+    def check(t0, t1, dt):
+        rc, n = _steps(t0, t1, dt)
+        ref = oracle_mod.num_steps(t0, t1, dt)
+        if 0 <= ref < 2147483647:
+            assert (rc, n) == (0, ref), (t0, t1, dt)
+        else:
+            assert rc == C.MR_ERR_STEP_COUNT, (t0, t1, dt, rc, n, ref)
+
+    check()
