@@ -307,6 +307,8 @@ def batch_cost(dev):
             "grid_sides": sorted({int(sc["bathy"]["x"].size) for sc in cfgs}), "rays": n_rays,
             "ray_steps": ray_steps, "ray_steps_bound": bound,
             "persistent_ms": t_batch, "persistent_ray_steps_per_s": ray_steps / (t_batch * 1e-3),
+            "frac_at_745_flop": ray_steps * FLOP_PER_RAY_STEP / (t_batch * 1e-3) / 1e12
+            / PEAK_FP64_TFLOPS,
             "graph_replay_ms": t_graph,
             "per_scene_launches_ms": t_scenes,
             "per_scene_ray_steps_per_s": ray_steps / (t_scenes * 1e-3),

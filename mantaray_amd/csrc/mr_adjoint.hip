@@ -64,31 +64,45 @@ hipError_t adjoint_scratch(void **ptr)
 
 // One RK4 step in fp64 mode from s: the first three stage derivatives k and
 // the new state o.  The arithmetic is that of trace_kernel's step, and so is
-// the cell prefetch: the stage points come in pairs at nearly one place
-// (y + k1 dt/2 with y + k2 dt/2, y + k3 dt with the next step's start), so
-// stages 1 and 3 start loading, into sn, the cells of the pair that follows
-// (guessed from the latest derivative (px, py)) and stages 2 and 4 take them
-// up; each evaluation uses the cells in hand.
-template <int BK, int CK, int MATH>
+// the cell scheme of its shared-cell step: two slots with fixed roles, sa
+// serving st 0 and st 3 (y and y + k3 dt, nearly the next step's start), sb
+// st 1 and st 2 (y + k1 dt/2 and y + k2 dt/2).  Each is refilled while the
+// other serves, two evaluations before its next use: sa at st 1 for the
+// point guessed at y + k1 dt, sb at st 3 for the next step's pair, guessed
+// at y + k3 (3 dt / 2); a lane loads only where the guess is not inside the
+// cells its slot holds, so no slot is ever copied.  Every evaluation tests
+// the cells it uses and queries again where they are not the target's, so
+// the slots choose only what is waited for, never a value.
+//
+// Before each refill the step waits for the vector memory operations issued
+// up to then but the last NST (the caller's stores after the step before,
+// issued after every load of it): the serving slot's loads, two evaluations
+// old, have landed by then.  Without that wait the compiler waits at the
+// serving slot's first use, after the refill, and as the refill is
+// conditional it cannot count it: it waits for the refill's loads as well.
+template <int BK, int CK, int MATH, int NST>
 __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs &BP,
                                            const Grid64View &C, const Grid64Ptrs &CP,
-                                           const Fields64 &F, const FastConsts &K, Slots64 &sn,
-                                           double &px, double &py, double dt, double half,
-                                           double sixth, const double s[4], double k[3][4],
-                                           double o[4])
+                                           const Fields64 &F, const FastConsts &K, Slots64 &sa,
+                                           Slots64 &sb, double dt, double half, double sixth,
+                                           const double s[4], double k[3][4], double o[4])
 {
     double sx = s[0], sy = s[1], skx = s[2], sky = s[3];
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
-    Slots64 sq = sn;
+    double px = 0.0, py = 0.0;                 // latest dx/dt, dy/dt, for the guesses
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const double hs = st < 2 ? half : dt;
-        if (st == 1 || st == 3)
-            sq = sn;
-        else
-            prefetch64<BK, CK>(B, BP, C, CP, F, sq, sn, s[0] + px * hs, s[1] + py * hs);
+        if (st == 1 || st == 3) {
+            if (st == 1) vm_wait_upto<NST>();
+            else vm_wait_upto<0>();
+            Slots64 &refill = st == 1 ? sa : sb;
+            const double g = st == 1 ? dt : 1.5 * dt;
+            prefetch64<BK, CK>(B, BP, C, CP, F, refill, refill, s[0] + px * g, s[1] + py * g);
+        }
         double dx, dy, dkx, dky;
-        rhs64<BK, CK, MATH>(B, BP, C, CP, F, sq, K, sx, sy, skx, sky, dx, dy, dkx, dky);
+        rhs64<BK, CK, MATH>(B, BP, C, CP, F, (st == 0 || st == 3) ? sa : sb, K, sx, sy, skx,
+                            sky, dx, dy, dkx, dky);
         px = dx; py = dy;
         if (st < 3) {
             k[st][0] = dx; k[st][1] = dy; k[st][2] = dkx; k[st][3] = dky;
@@ -108,9 +122,13 @@ __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs
     o[2] = s[2] + akx * sixth; o[3] = s[3] + aky * sixth;
 }
 
+// stores a recomputing wave issues after each step: its state and three
+// stage derivatives
+constexpr int RECOMPUTE_STORES = 16;
+
 // ---------------------------------------------------------------------------
-// fp64 trace: the loop of trace_kernel (mr_kernels.hip) with rhs64 and
-// without record prefetch (every query reads its corners).  Checkpoint c
+// fp64 trace: the loop of trace_kernel (mr_kernels.hip) with rhs64 and the
+// two fixed-role cell slots of rk4_step64.  Checkpoint c
 // holds the state at row c * stride, stored when the step from that row
 // starts; a ray's checkpoints up to its last live step are always written.
 
@@ -137,15 +155,16 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
     const Grid64View C = load_view<Grid64View>(a.C);
     const FastConsts K = load_view<FastConsts>(mr_fast_consts);
     const double half = a.half, dt = a.dt, sixth = a.sixth;
-    Slots64 sn;                        // cells for the next evaluation
-    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, Slots64{}, sn, s[0], s[1]);
+    Slots64 sa, sb;                    // the step's two cell slots
+    load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
+    load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
     // all of it in registers before the loop, so that the loop head has
     // nothing pending to wait for (as in trace_kernel)
     settle(B);
     settle(C);
     settle(K);
-    settle(sn);
-    double px = 0.0, py = 0.0;         // latest dx/dt, dy/dt, for the guess
+    settle(sa);
+    settle(sb);
     bool alive = true;
     int64_t len = 1;
     int64_t s_i = 1;
@@ -164,8 +183,10 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
         double o[4] = {MR_NAN, MR_NAN, MR_NAN, MR_NAN};
         if (alive) {
             double k[3][4], n[4];
-            rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half, sixth, s,
-                                     k, n);
+            // the row stores of the step before are the last operations
+            // issued before this step's first refill
+            rk4_step64<BK, CK, MATH, OUT == 1 ? 4 : 0>(B, a.bp, C, a.cp, a.f, K, sa, sb, dt,
+                                                       half, sixth, s, k, n);
             // the MR_SOLOUT_NEXT_DERIV test applies from the second step on
             // (solout is asked only after a step)
             if (a.solout == 1 && s_i > 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
@@ -317,8 +338,8 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
     // segments 0 .. cmax_r of this ray; rounds over the workgroup's largest
     const int cmax_r = (active && m >= 1) ? (int)((m - 1) / stride) : -1;
     const int cmax = wave_max(cmax_r);
-    Slots64 sn;                                        // cells for the next evaluation
     if (wave == 0) {
+        Slots64 sa, sb;                                // the step's two cell slots
         for (int t = 0; t <= cmax + 1; ++t) {
             const int c = cmax - t;
             if (c >= 0 && c <= cmax_r) {
@@ -327,12 +348,12 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
                 double *wk = a.work + (t & 1) * half_work + r;
                 const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
                 double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
-                prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
-                double px = 0.0, py = 0.0;
+                load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
+                load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
                 for (int j = 0; j < nseg; ++j) {
                     double k[3][4], n[4];
-                    rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
-                                             sixth, s, k, n);
+                    rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb,
+                                                               dt, half, sixth, s, k, n);
                     double *w = wk + (int64_t)j * 16 * ld;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -360,6 +381,7 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
         if (a.g_kx) lam[2] += a.g_kx[o];
         if (a.g_ky) lam[3] += a.g_ky[o];
     }
+    Slots64 sn;                                        // cells for the next evaluation
     Accs64 acc;
     for (int t = 0; t <= cmax + 1; ++t) {
         const int c = cmax - t + 1;
@@ -450,7 +472,9 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
         if (a.g_kx) lam[2] += a.g_kx[o];
         if (a.g_ky) lam[3] += a.g_ky[o];
     }
-    Slots64 sn;                                        // cells for the next evaluation
+    // the recomputation's two cell slots; the sweep then uses sa (both are
+    // loaded whole at each segment's start)
+    Slots64 sa, sb;
     Accs64 acc;
     double *wk = a.work + r;                           // the first half of the buffer
     for (int c = cmax; c >= 0; --c) {
@@ -460,12 +484,12 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
             // recompute the segment from its checkpoint
             const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
             double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
-            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
-            double px = 0.0, py = 0.0;
+            load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
+            load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
             for (int j = 0; j < nseg; ++j) {
                 double k[3][4], n[4];
-                rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
-                                         sixth, s, k, n);
+                rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb, dt,
+                                                           half, sixth, s, k, n);
                 double *w = wk + (int64_t)j * 16 * ld;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -495,9 +519,9 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
                     for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
-                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, y[0] + k[2][0] * dt,
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, sa, y[0] + k[2][0] * dt,
                                        y[1] + k[2][1] * dt);
-                rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sn, a.grads, acc, K, dt, half,
+                rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sa, a.grads, acc, K, dt, half,
                                          sixth, y, k, lam);
                 const int64_t o = (i0 + j) * nr + r;
                 if (a.g_x) lam[0] += a.g_x[o];
@@ -539,6 +563,15 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
 // the sweep is latency-bound (the LDS ring holds one workgroup per CU).  NL
 // = 2 is the one launched: with NL = 4 the six waves share SIMDs and the
 // 256 registers a wave then gets spill.
+// Whether linearising wave W (NL = 2: 1 or 2; 0: every wave otherwise)
+// uses field f of a step's data in the work buffer: f in [0, 4) is the
+// state y, [4, 12) the stage derivatives k1 and k2, [12, 16) k3.
+template <int W>
+__device__ constexpr bool lin_uses(int f)
+{
+    return f < 4 || (f < 12 ? W != 2 : W != 1);
+}
+
 template <int BK, int CK, int MATH, int NL>
 __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs a)
 {
@@ -568,9 +601,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
     // per round: the state of one role is then not live across another's
     // code, and registers go to the largest role, not to all of them.
     if (wave == 0) {
-        Slots64 sn;                                    // cells for the next evaluation
+        Slots64 sa, sb;                                // the step's two cell slots
         double s[4] = {0.0, 0.0, 0.0, 0.0};            // the state being advanced
-        double px = 0.0, py = 0.0;
         for (int rho = 0; rho < rounds; ++rho) {
             const int c = cmax - rho / stride, j = rho % stride;
             if (c >= 0 && c <= cmax_r) {
@@ -579,12 +611,12 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     if (j == 0) {
                         const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
                         s[0] = cp[0]; s[1] = cp[nr]; s[2] = cp[2 * nr]; s[3] = cp[3 * nr];
-                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, s[0], s[1]);
-                        px = 0.0; py = 0.0;
+                        load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
+                        load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
                     }
                     double k[3][4], n[4];
-                    rk4_step64<BK, CK, MATH>(B, a.bp, C, a.cp, a.f, K, sn, px, py, dt, half,
-                                             sixth, s, k, n);
+                    rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa,
+                                                               sb, dt, half, sixth, s, k, n);
                     double *w = a.work + (c & 1) * half_work + r + (int64_t)j * 16 * ld;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -599,115 +631,131 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
             __syncthreads();
         }
     } else if (wave < chain) {
-        Slots64 sn;
-        double nxt[16];                                // the next step's data
-        bool pre = false;
-        for (int rho = 0; rho < rounds; ++rho) {
-            const int qq = rho / stride;
-            const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
-            if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
-                const double *wb = a.work + (c & 1) * half_work + r;
-                double d[16];
-                if (pre) {
+        // W: the linearising wave with NL = 2 (1 or 2), as a compile-time
+        // value, so that each loads and keeps only the step data its stages
+        // use (y with k1 and k2 for wave 1, y with k3 for wave 2); 0 for the
+        // general case
+        auto lin_role = [&](auto wc) {
+            constexpr int W = decltype(wc)::value;
+            constexpr bool use_k12 = W != 2, use_k3 = W != 1;
+            Slots64 sn;
+            double nxt[16];                                // the next step's data
+            bool pre = false;
+            for (int rho = 0; rho < rounds; ++rho) {
+                const int qq = rho / stride;
+                const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
+                if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
+                    const double *wb = a.work + (c & 1) * half_work + r;
+                    double d[16];
+                    if (pre) {
 #pragma unroll
-                    for (int f = 0; f < 16; ++f) d[f] = nxt[f];
-                } else {
-                    const double *w = wb + (int64_t)j * 16 * ld;
-#pragma unroll
-                    for (int f = 0; f < 16; ++f) d[f] = w[f * ld];
-                }
-                // start loading the next round's step (same segment)
-                pre = j >= 1;
-                if (pre) {
-                    const double *w = wb + (int64_t)(j - 1) * 16 * ld;
-#pragma unroll
-                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
-                }
-                // the four stage points y, y + k1 dt/2, y + k2 dt/2, y + k3 dt
-                double P[4][4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    P[0][q] = d[q];
-                    P[1][q] = d[q] + d[4 + q] * half;
-                    P[2][q] = d[q] + d[8 + q] * half;
-                    P[3][q] = d[q] + d[12 + q] * dt;
-                }
-                if (NL == 2) {
-                    // The stage points pair up: stages 2 and 1 lie together near
-                    // y + k dt/2, and stage 0 here lies with stage 3 of the next
-                    // round (one step earlier).  Wave 1 takes stages 2 and 1,
-                    // wave 2 stages 3 and 0, so each needs one prefetch per
-                    // round: wave 1 for the next round's midpoint (guessed one
-                    // step back along k1), wave 2 for stage 0 (known), which
-                    // it evaluates in the prefetched slot (the lanes that did
-                    // not prefetch hold the same cells in both).
-                    Slots64 sq = sn;
-                    double R[LIN_N];
-                    if (wave == 1) {
-                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] - d[4] * dt,
-                                           P[1][1] - d[5] * dt);
-                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
-                                          P[2][2], P[2][3], R);
-#pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][2][f][lane] = R[f];
-                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[1][0], P[1][1],
-                                          P[1][2], P[1][3], R);
-#pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][1][f][lane] = R[f];
+                        for (int f = 0; f < 16; ++f)
+                            if (lin_uses<W>(f)) d[f] = nxt[f];
                     } else {
-                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
-                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[3][0], P[3][1],
-                                          P[3][2], P[3][3], R);
+                        const double *w = wb + (int64_t)j * 16 * ld;
 #pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][3][f][lane] = R[f];
-                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sn, K, P[0][0], P[0][1],
-                                          P[0][2], P[0][3], R);
+                        for (int f = 0; f < 16; ++f)
+                            if (lin_uses<W>(f)) d[f] = w[f * ld];
+                    }
+                    // start loading the next round's step (same segment)
+                    pre = j >= 1;
+                    if (pre) {
+                        const double *w = wb + (int64_t)(j - 1) * 16 * ld;
 #pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][0][f][lane] = R[f];
+                        for (int f = 0; f < 16; ++f)
+                            if (lin_uses<W>(f)) nxt[f] = w[f * ld];
+                    }
+                    // the four stage points y, y + k1 dt/2, y + k2 dt/2, y + k3 dt
+                    double P[4][4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        P[0][q] = d[q];
+                        if (use_k12) {
+                            P[1][q] = d[q] + d[4 + q] * half;
+                            P[2][q] = d[q] + d[8 + q] * half;
+                        }
+                        if (use_k3) P[3][q] = d[q] + d[12 + q] * dt;
+                    }
+                    if (NL == 2) {
+                        // The stage points pair up: stages 2 and 1 lie together near
+                        // y + k dt/2, and stage 0 here lies with stage 3 of the next
+                        // round (one step earlier).  Wave 1 takes stages 2 and 1,
+                        // wave 2 stages 3 and 0, so each needs one prefetch per
+                        // round: wave 1 for the next round's midpoint (guessed one
+                        // step back along k1), wave 2 for stage 0 (known), which
+                        // it evaluates in the prefetched slot (the lanes that did
+                        // not prefetch hold the same cells in both).
+                        Slots64 sq = sn;
+                        double R[LIN_N];
+                        if constexpr (W == 1) {
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] - d[4] * dt,
+                                               P[1][1] - d[5] * dt);
+                            rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
+                                              P[2][2], P[2][3], R);
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][2][f][lane] = R[f];
+                            rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[1][0], P[1][1],
+                                              P[1][2], P[1][3], R);
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][1][f][lane] = R[f];
+                        } else {
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
+                            rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[3][0], P[3][1],
+                                              P[3][2], P[3][3], R);
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][3][f][lane] = R[f];
+                            rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sn, K, P[0][0], P[0][1],
+                                              P[0][2], P[0][3], R);
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][0][f][lane] = R[f];
+                        }
+                    } else {
+                        // this wave's stages, latest first (the wave selects values,
+                        // the indices stay compile-time)
+                        const int first = 3 - (wave - 1) * SPW;
+#pragma unroll
+                        for (int h = 0; h < SPW; ++h) {
+                            const int st = first - h;
+                            double Y[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+                                Y[q] = st == 3   ? P[3][q]
+                                       : st == 2 ? P[2][q]
+                                       : st == 1 ? P[1][q]
+                                                 : P[0][q];
+                            Slots64 sq = sn;
+                            // the next point: this wave's next stage, or, after its
+                            // last, its first stage one step earlier (guessed one
+                            // step back along k1)
+                            double gx, gy;
+                            if (h + 1 < SPW) {
+                                const int sx_ = st - 1;
+                                gx = sx_ == 2 ? P[2][0] : sx_ == 1 ? P[1][0] : P[0][0];
+                                gy = sx_ == 2 ? P[2][1] : sx_ == 1 ? P[1][1] : P[0][1];
+                            } else {
+                                const double fx = first == 3 ? P[3][0] : first == 2 ? P[2][0]
+                                                : first == 1 ? P[1][0] : P[0][0];
+                                const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
+                                                : first == 1 ? P[1][1] : P[0][1];
+                                gx = fx - d[4] * dt;
+                                gy = fy - d[5] * dt;
+                            }
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
+                            double R[LIN_N];
+                            rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
+                        }
                     }
                 } else {
-                    // this wave's stages, latest first (the wave selects values,
-                    // the indices stay compile-time)
-                    const int first = 3 - (wave - 1) * SPW;
-#pragma unroll
-                    for (int h = 0; h < SPW; ++h) {
-                        const int st = first - h;
-                        double Y[4];
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            Y[q] = st == 3   ? P[3][q]
-                                   : st == 2 ? P[2][q]
-                                   : st == 1 ? P[1][q]
-                                             : P[0][q];
-                        Slots64 sq = sn;
-                        // the next point: this wave's next stage, or, after its
-                        // last, its first stage one step earlier (guessed one
-                        // step back along k1)
-                        double gx, gy;
-                        if (h + 1 < SPW) {
-                            const int sx_ = st - 1;
-                            gx = sx_ == 2 ? P[2][0] : sx_ == 1 ? P[1][0] : P[0][0];
-                            gy = sx_ == 2 ? P[2][1] : sx_ == 1 ? P[1][1] : P[0][1];
-                        } else {
-                            const double fx = first == 3 ? P[3][0] : first == 2 ? P[2][0]
-                                            : first == 1 ? P[1][0] : P[0][0];
-                            const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
-                                            : first == 1 ? P[1][1] : P[0][1];
-                            gx = fx - d[4] * dt;
-                            gy = fy - d[5] * dt;
-                        }
-                        prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
-                        double R[LIN_N];
-                        rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
-#pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
-                    }
+                    pre = false;
                 }
-            } else {
-                pre = false;
+                __syncthreads();
             }
-            __syncthreads();
-        }
+        };
+        if (NL == 2 && wave == 1) lin_role(std::integral_constant<int, 1>{});
+        else if (NL == 2) lin_role(std::integral_constant<int, 2>{});
+        else lin_role(std::integral_constant<int, 0>{});
     } else {
         double lam[4] = {0.0, 0.0, 0.0, 0.0};          // the cotangent
         Accs64 acc;

@@ -290,6 +290,27 @@ struct Slots64 {
     Slot64<2> c;
 };
 
+// The cells that (x, y) most likely lies in, loaded into s whatever it
+// holds (a ray's first cells, or a segment's).
+template <int BK, int CK>
+__device__ __forceinline__ void load_guess64(const Grid64View &B, const Grid64Ptrs &BP,
+                                             const Grid64View &C, const Grid64Ptrs &CP,
+                                             const Fields64 &F, Slots64 &s, double x, double y)
+{
+    if (BK == BK_GRID) slot64_prefetch<1>(B, BP, F.depth, F.depth, s.b, x, y);
+    if (CK == CK_GRID) slot64_prefetch<2>(C, CP, F.u, F.v, s.c, x, y);
+}
+
+// s_waitcnt on the vector memory count alone, for counts up to 63 (gfx9
+// splits vmcnt over bits 3:0 and 15:14; expcnt and lgkmcnt at their no-wait
+// maxima).
+template <int N>
+__device__ __forceinline__ void vm_wait_upto()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt below 64");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+}
+
 // Start loading, into n, the cells of (x, y) that q does not hold.
 template <int BK, int CK>
 __device__ __forceinline__ void prefetch64(const Grid64View &B, const Grid64Ptrs &BP,
