@@ -300,6 +300,21 @@ std::vector<double2> axis_cells64(const std::vector<double> &a)
     return c;
 }
 
+// per-cell fast thresholds of the fp64 mode: the f64 index's cell
+// [T[i], T[i+1]) narrowed to the open node interval (a[i], a[i+1]), so that a
+// target inside on both axes is in the cell and on none of its node lines
+// (an interval that comes out empty holds no target: every target of that
+// cell then takes the full query)
+std::vector<double2> axis_fast64(const std::vector<double> &a)
+{
+    const std::vector<double> t = thresholds64(a);
+    std::vector<double2> c(a.size() - 1);
+    for (size_t i = 0; i + 1 < a.size(); i++)
+        c[i] = make_double2(std::max(t[i], std::nextafter(a[i], INFINITY)),
+                            std::min(t[i + 1], a[i + 1]));
+    return c;
+}
+
 std::vector<float2> axis_pairs(const std::vector<float> &a)
 {
     std::vector<float2> p(a.size() - 1);
@@ -519,9 +534,9 @@ int build64(const mr_field &f, DevTables &t)
         v.gx0 = (float)ax[0]; v.gy0 = (float)ay[0];
         v.ginv_x = (float)v.inv_sx; v.ginv_y = (float)v.inv_sy;
         v.gmax_x = (float)(f.nx - 2); v.gmax_y = (float)(f.ny - 2);
-        // the cell thresholds of the f64 index (p - a0)/|a1 - a0| on these
+        // the fast thresholds of the f64 index (p - a0)/|a1 - a0| on these
         // axes, and the node pairs of each cell
-        std::vector<double2> xt = axis_cells64(ax), yt = axis_cells64(ay);
+        std::vector<double2> xt = axis_fast64(ax), yt = axis_fast64(ay);
         std::vector<double2> xp(ax.size() - 1), yp(ay.size() - 1);
         for (size_t i = 0; i + 1 < ax.size(); i++) xp[i] = make_double2(ax[i], ax[i + 1]);
         for (size_t i = 0; i + 1 < ay.size(); i++) yp[i] = make_double2(ay[i], ay[i + 1]);

@@ -23,9 +23,9 @@
 // so a caller can differentiate with respect to them and change them between
 // calls without rebuilding anything; what depends only on the axes (cell
 // thresholds, node pairs, the change of basis m01, m10) is tabulated once per
-// field.  As in the reference mode, each lane holds the cell it is in (a
-// Slot64) and the cell its next evaluation will most likely need, loaded one
-// evaluation ahead.
+// field.  As in the reference mode, each lane holds cells in registers (a
+// Slot64 per field) and loads the cell a coming evaluation will most likely
+// need ahead of it.
 //
 // Linearisation.  With u, w the bilinear fractions of the query, the value is
 //   v = wA A + wB B + wC C + wD D,  wA = (1-u)(1-w), wB = u(1-w), wC = u w,
@@ -69,7 +69,8 @@ struct Grid64View {
 
 // Device arrays of a field in fp64 mode (kernel arguments).
 struct Grid64Ptrs {
-    const double2 *xt, *yt;               // per axis cell i: (T[i], T[i+1]) cell thresholds
+    const double2 *xt, *yt;               // per axis cell i: fast thresholds, [T[i], T[i+1])
+                                          // narrowed to the open node interval
     const double2 *xp, *yp;               // per axis cell i: (a[i], a[i+1]) in f64
     const double2 *m;                     // per cell, row-major: (m01, m10); NaN m01 marks det == 0
     const float *arr;                     // ArrayDepth values
@@ -89,11 +90,17 @@ struct Cell64 {
     double u, w;
 };
 
-// One cell of a gridded field held per lane: its thresholds (the exact cell
-// test, as in the reference mode), node coordinates, change of basis, and
-// the four corners of NV value arrays (depth; or u and v).  valid is false
-// for a cell with a corner past the data and for a slot never loaded;
-// thresholds start as NaN, so such a slot contains no point.
+// One cell of a gridded field held per lane: its fast thresholds, node
+// coordinates, change of basis, and the four corners of NV value arrays
+// (depth; or u and v).  The fast thresholds of cell i on an axis are the
+// f64 index's cell [T[i], T[i+1]) narrowed to the open interval between the
+// cell's nodes (a[i], a[i+1]): a target inside them on both axes is in the
+// cell and on none of its node lines, so the bilinear value needs neither
+// the corner exit nor the node case of the det == 0 exit.  Targets of the
+// cell on a node line (rare: the node coordinates themselves) take the full
+// query, which locates them by the index.  valid is false for a cell with a
+// corner past the data and for a slot never loaded; thresholds start as NaN,
+// so such a slot contains no point.
 template <int NV>
 struct Slot64 {
     double2 tx = {MR_NAN_INIT, MR_NAN_INIT}, ty = {MR_NAN_INIT, MR_NAN_INIT};
@@ -184,9 +191,10 @@ __device__ __forceinline__ bool slot64_query(const Grid64View &G, const Grid64Pt
 
 // interpolator::bilinear in f64 on the slot's cell for value array q, with
 // its corner exit (a target on a node returns that corner, first of A, B,
-// C, D), and the cell-difference slopes.  The det == 0 exit is the
-// caller's (m01 is NaN there).
-template <int NV>
+// C, D; EXITS false leaves it out, for a target inside the slot's fast
+// thresholds, which is on no node), and the cell-difference slopes.  The
+// det == 0 exit is the caller's (m01 is NaN there).
+template <int NV, bool EXITS = true>
 __device__ __forceinline__ Val64 slot64_value(const Grid64View &G, const Slot64<NV> &s, int q,
                                               double x, double y, Cell64 &c)
 {
@@ -198,9 +206,11 @@ __device__ __forceinline__ Val64 slot64_value(const Grid64View &G, const Slot64<
     c.w = s.m.y * (x - s.xp.x);
     Val64 r;
     r.val = ((vA + a10 * c.u) + a01 * c.w) + (a11 * c.u) * c.w;
-    const bool ex = x == s.xp.x, exd = x == s.xp.y, eya = y == s.yp.x, eyb = y == s.yp.y;
-    if ((ex | exd) & (eya | eyb))
-        r.val = ex ? (eya ? vA : vB) : (eyb ? vC : vD);
+    if (EXITS) {
+        const bool ex = x == s.xp.x, exd = x == s.xp.y, eya = y == s.yp.x, eyb = y == s.yp.y;
+        if ((ex | exd) & (eya | eyb))
+            r.val = ex ? (eya ? vA : vB) : (eyb ? vC : vD);
+    }
     r.gx = (vD - vA) * G.rxs;
     r.gy = (vB - vA) * G.rys;
     r.dx = (a01 + a11 * c.u) * s.m.y;
@@ -208,10 +218,12 @@ __device__ __forceinline__ Val64 slot64_value(const Grid64View &G, const Slot64<
     return r;
 }
 
-// the det == 0 exit: m01 is NaN, unless the target is on a node
-template <int NV>
+// the det == 0 exit: m01 is NaN, unless the target is on a node (EXITS
+// false: a target inside the fast thresholds, on no node)
+template <int NV, bool EXITS = true>
 __device__ __forceinline__ bool slot64_det_ok(const Slot64<NV> &s, double x, double y)
 {
+    if (!EXITS) return !__builtin_isnan(s.m.x);
     const bool node = ((x == s.xp.x) | (x == s.xp.y)) & ((y == s.yp.x) | (y == s.yp.y));
     return node | !__builtin_isnan(s.m.x);
 }
@@ -342,16 +354,16 @@ __device__ __forceinline__ void rhs64(const Grid64View &B, const Grid64Ptrs &BP,
     bool slow = false;
     if (BK == BK_GRID) {
         slow = !slot64_inside<1>(sl.b, x, y);
-        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
-        okb = slot64_det_ok<1>(sl.b, x, y);
+        h = slot64_value<1, false>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1, false>(sl.b, x, y);
     } else {
         okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
     }
     if (CK == CK_GRID) {
         slow = slow | !slot64_inside<2>(sl.c, x, y);
-        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
-        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
-        okc = slot64_det_ok<2>(sl.c, x, y);
+        u = slot64_value<2, false>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2, false>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2, false>(sl.c, x, y);
     } else {
         okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     }
@@ -519,16 +531,16 @@ __device__ __forceinline__ void rhs64_lin(const Grid64View &B, const Grid64Ptrs 
     bool slow = false;
     if (BK == BK_GRID) {
         slow = !slot64_inside<1>(sl.b, x, y);
-        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
-        okb = slot64_det_ok<1>(sl.b, x, y);
+        h = slot64_value<1, false>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1, false>(sl.b, x, y);
     } else {
         okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
     }
     if (CK == CK_GRID) {
         slow = slow | !slot64_inside<2>(sl.c, x, y);
-        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
-        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
-        okc = slot64_det_ok<2>(sl.c, x, y);
+        u = slot64_value<2, false>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2, false>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2, false>(sl.c, x, y);
     } else {
         okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     }
@@ -595,16 +607,16 @@ __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs 
     bool slow = false;
     if (BK == BK_GRID) {
         slow = !slot64_inside<1>(sl.b, x, y);
-        h = slot64_value<1>(B, sl.b, 0, x, y, bc);
-        okb = slot64_det_ok<1>(sl.b, x, y);
+        h = slot64_value<1, false>(B, sl.b, 0, x, y, bc);
+        okb = slot64_det_ok<1, false>(sl.b, x, y);
     } else {
         okb = bathy64<BK>(B, BP, F.depth, sl.b, x, y, h, bc);
     }
     if (CK == CK_GRID) {
         slow = slow | !slot64_inside<2>(sl.c, x, y);
-        u = slot64_value<2>(C, sl.c, 0, x, y, cc);
-        v = slot64_value<2>(C, sl.c, 1, x, y, cc);
-        okc = slot64_det_ok<2>(sl.c, x, y);
+        u = slot64_value<2, false>(C, sl.c, 0, x, y, cc);
+        v = slot64_value<2, false>(C, sl.c, 1, x, y, cc);
+        okc = slot64_det_ok<2, false>(sl.c, x, y);
     } else {
         okc = current64<CK>(C, CP, F.u, F.v, sl.c, x, y, u, v, cc);
     }
