@@ -782,11 +782,21 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                         kb[0][q] = w; kb[3][q] = w;
                         kb[1][q] = 2.0 * w; kb[2][q] = 2.0 * w;
                     }
+                    // each stage's record is read while the stage before it
+                    // in the chain is applied, so the LDS latency is hidden
+                    double Rn[LIN_N];
+#pragma unroll
+                    for (int f = 0; f < LIN_N; ++f) Rn[f] = ring[(rho - 1) & 1][3][f][lane];
 #pragma unroll
                     for (int st = 3; st >= 0; --st) {
                         double R[LIN_N];
 #pragma unroll
-                        for (int f = 0; f < LIN_N; ++f) R[f] = ring[(rho - 1) & 1][st][f][lane];
+                        for (int f = 0; f < LIN_N; ++f) R[f] = Rn[f];
+                        if (st > 0) {
+#pragma unroll
+                            for (int f = 0; f < LIN_N; ++f)
+                                Rn[f] = ring[(rho - 1) & 1][st - 1][f][lane];
+                        }
                         double z[4];
                         lin_apply<BK, CK>(R, B, C, a.grads, acc, kb[st], z);
                         if (st > 0) {
