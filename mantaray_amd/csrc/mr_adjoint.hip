@@ -693,21 +693,25 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
                                               P[2][2], P[2][3], R);
 #pragma unroll
-                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][2][f][lane] = R[f];
+                            for (int f = 0; f < LIN_N; ++f)
+                                if (lin_field_used<BK, CK>(f)) ring[rho & 1][2][f][lane] = R[f];
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[1][0], P[1][1],
                                               P[1][2], P[1][3], R);
 #pragma unroll
-                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][1][f][lane] = R[f];
+                            for (int f = 0; f < LIN_N; ++f)
+                                if (lin_field_used<BK, CK>(f)) ring[rho & 1][1][f][lane] = R[f];
                         } else {
                             prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[3][0], P[3][1],
                                               P[3][2], P[3][3], R);
 #pragma unroll
-                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][3][f][lane] = R[f];
+                            for (int f = 0; f < LIN_N; ++f)
+                                if (lin_field_used<BK, CK>(f)) ring[rho & 1][3][f][lane] = R[f];
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sn, K, P[0][0], P[0][1],
                                               P[0][2], P[0][3], R);
 #pragma unroll
-                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][0][f][lane] = R[f];
+                            for (int f = 0; f < LIN_N; ++f)
+                                if (lin_field_used<BK, CK>(f)) ring[rho & 1][0][f][lane] = R[f];
                         }
                     } else {
                         // this wave's stages, latest first (the wave selects values,
@@ -744,7 +748,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                             double R[LIN_N];
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, Y[0], Y[1], Y[2], Y[3], R);
 #pragma unroll
-                            for (int f = 0; f < LIN_N; ++f) ring[rho & 1][st][f][lane] = R[f];
+                            for (int f = 0; f < LIN_N; ++f)
+                                if (lin_field_used<BK, CK>(f)) ring[rho & 1][st][f][lane] = R[f];
                         }
                     }
                 } else {
@@ -786,7 +791,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     // in the chain is applied, so the LDS latency is hidden
                     double Rn[LIN_N];
 #pragma unroll
-                    for (int f = 0; f < LIN_N; ++f) Rn[f] = ring[(rho - 1) & 1][3][f][lane];
+                    for (int f = 0; f < LIN_N; ++f)
+                        Rn[f] = lin_field_used<BK, CK>(f) ? ring[(rho - 1) & 1][3][f][lane] : 0.0;
 #pragma unroll
                     for (int st = 3; st >= 0; --st) {
                         double R[LIN_N];
@@ -795,7 +801,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                         if (st > 0) {
 #pragma unroll
                             for (int f = 0; f < LIN_N; ++f)
-                                Rn[f] = ring[(rho - 1) & 1][st - 1][f][lane];
+                                Rn[f] = lin_field_used<BK, CK>(f)
+                                            ? ring[(rho - 1) & 1][st - 1][f][lane] : 0.0;
                         }
                         double z[4];
                         lin_apply<BK, CK>(R, B, C, a.grads, acc, kb[st], z);

@@ -490,6 +490,29 @@ __device__ __forceinline__ void unpack_cells(double v, uint32_t &b, uint32_t &c)
     c = (uint32_t)(u >> 32);
 }
 
+// Whether the depth has no gradient anywhere (ConstantDepth, ArrayDepth:
+// their queries return zero slopes and zero derivatives), so that every
+// depth-gradient term of the linearisation vanishes.
+template <int BK>
+__device__ constexpr bool depth_is_flat()
+{
+    return BK == BK_CONSTANT || BK == BK_ARRAY;
+}
+
+// Whether lin_apply reads field f of a linearisation record for the field
+// kinds BK, CK (the only fields the ring is given): the depth terms (16-20,
+// 23, 24) for gridded depth, the current's (21, 22, 25, 26) for a gridded
+// current, and with a flat depth not the four zeros of M's x and y rows.
+template <int BK, int CK>
+__device__ constexpr bool lin_field_used(int f)
+{
+    if (f < 16)
+        return !depth_is_flat<BK>() || f == 0 || f == 1 || f == 4 || f == 5 || f >= 8;
+    if (f == 27) return true;
+    if (f <= 20 || f == 23 || f == 24) return BK == BK_GRID;
+    return CK == CK_GRID;
+}
+
 // The linearisation record of one evaluation from its field values (the
 // arithmetic of rhs64_lin below).
 template <int BK, int CK>
@@ -516,6 +539,26 @@ __device__ __forceinline__ void lin_record(const FastConsts &K, const Val64 &h, 
     const double iope = em * D;
     const double T = em * iope;
     const double S = (4.0 * e) * (iope * iope);
+    if (depth_is_flat<BK>()) {
+        // no depth gradient: the rows of M are the current's and the
+        // c_g k/|k| terms only
+        const double rs = W * ope;
+        const double N = __builtin_fma(q, S, T);
+        const double cg = K.half_g * N * rs;
+        const double Nq = 2.0 * S * (1.0 - q * T);
+        const double cg_k = K.half_g * rs * (hh * Nq - N * cg * rs);
+        const double tk = cg * rk;
+        const double a = cg_k - tk;
+        R[0] = u.dx; R[1] = v.dx; R[2] = 0.0; R[3] = 0.0;
+        R[4] = u.dy; R[5] = v.dy; R[6] = 0.0; R[7] = 0.0;
+        R[8] = a * ex * ex + tk; R[9] = a * ex * ey; R[10] = -u.gx; R[11] = -u.gy;
+        R[12] = a * ey * ex; R[13] = a * ey * ey + tk; R[14] = -v.gx; R[15] = -v.gy;
+        R[16] = 0.0; R[17] = 0.0; R[18] = 0.0; R[19] = 0.0; R[20] = 0.0;
+        R[21] = kx; R[22] = ky;
+        R[23] = 0.0; R[24] = 0.0; R[25] = cc.u; R[26] = cc.w;
+        R[27] = pack_cells(0xFFFFFFFFu, (CK == CK_GRID && okc) ? cc.iA : 0xFFFFFFFFu);
+        return;
+    }
     const double invT = (ope * ope) * D;
     const double gkT = gk * T;
     const double rs = W * ope;
@@ -597,9 +640,13 @@ __device__ __forceinline__ void lin_apply(const double R[LIN_N], const Grid64Vie
                                           const double g[4], double z[4])
 {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        z[i] = ((R[4 * i] * g[0] + R[4 * i + 1] * g[1]) + R[4 * i + 2] * g[2]) +
-               R[4 * i + 3] * g[3];
+    for (int i = 0; i < 4; ++i) {
+        if (depth_is_flat<BK>() && i < 2)      // R[2], R[3], R[6], R[7] are zero
+            z[i] = R[4 * i] * g[0] + R[4 * i + 1] * g[1];
+        else
+            z[i] = ((R[4 * i] * g[0] + R[4 * i + 1] * g[1]) + R[4 * i + 2] * g[2]) +
+                   R[4 * i + 3] * g[3];
+    }
     uint32_t ib, ic;
     unpack_cells(R[27], ib, ic);
     if (BK == BK_GRID && Gr.depth && ib != 0xFFFFFFFFu) {
@@ -674,32 +721,39 @@ __device__ __forceinline__ void rhs64_vjp(const Grid64View &B, const Grid64Ptrs 
     const double iope = em * D;
     const double T = em * iope;                         // tanh q
     const double S = (4.0 * e) * (iope * iope);         // sech^2 q
-    const double invT = (ope * ope) * D;                // 1/tanh q
     const double gkT = (K.g * k) * T;
     const double rs = rsq_nr(gkT);                      // 1/sigma
-    const double sigma = gkT * rs;
     const double N = __builtin_fma(q, S, T);
     const double cg = K.half_g * N * rs;
-    const double c = (-2.0 * k) * e * sigma * D;
-    const double sig_h = K.half_g * (k * k) * S * rs;   // d sigma / dh
     const double Nq = 2.0 * S * (1.0 - q * T);
     const double cg_k = K.half_g * rs * (hh * Nq - N * cg * rs);
-    const double cg_h = K.half_g * rs * (k * Nq - N * sig_h * rs);
-    const double coth2 = 0.5 * (T + invT);
-    const double c_k = c * (rk + cg * rs - 2.0 * hh * coth2);
-    const double c_h = c * (sig_h * rs - 2.0 * k * coth2);
-
     const double P = g0 * ex + g1 * ey;                 // cotangent of cg
-    const double cbar = g2 * h.gx + g3 * h.gy;          // cotangent of c
-    const double hbar = cg_h * P + c_h * cbar;
     const double tk = cg * rk;
-    bkx += (cg_k * P) * ex + tk * (g0 - P * ex) + (c_k * cbar) * ex - g2 * u.gx - g3 * u.gy;
-    bky += (cg_k * P) * ey + tk * (g1 - P * ey) + (c_k * cbar) * ey - g2 * v.gx - g3 * v.gy;
-    bx += hbar * h.dx + g0 * u.dx + g1 * v.dx;
-    by += hbar * h.dy + g0 * u.dy + g1 * v.dy;
-    // (a state of a finite row never errs; the guards keep a misuse in bounds)
-    if (BK == BK_GRID && Gr.depth && okb)
-        acc_add(acc.h, Gr.depth, (uint32_t)B.nx, bc, B.rxs, B.rys, hbar, g2 * c, g3 * c);
+    if (depth_is_flat<BK>()) {
+        // no depth gradient: the terms of c and of the depth cotangent vanish
+        bkx += (cg_k * P) * ex + tk * (g0 - P * ex) - g2 * u.gx - g3 * u.gy;
+        bky += (cg_k * P) * ey + tk * (g1 - P * ey) - g2 * v.gx - g3 * v.gy;
+        bx += g0 * u.dx + g1 * v.dx;
+        by += g0 * u.dy + g1 * v.dy;
+    } else {
+        const double invT = (ope * ope) * D;            // 1/tanh q
+        const double sigma = gkT * rs;
+        const double c = (-2.0 * k) * e * sigma * D;
+        const double sig_h = K.half_g * (k * k) * S * rs;   // d sigma / dh
+        const double cg_h = K.half_g * rs * (k * Nq - N * sig_h * rs);
+        const double coth2 = 0.5 * (T + invT);
+        const double c_k = c * (rk + cg * rs - 2.0 * hh * coth2);
+        const double c_h = c * (sig_h * rs - 2.0 * k * coth2);
+        const double cbar = g2 * h.gx + g3 * h.gy;      // cotangent of c
+        const double hbar = cg_h * P + c_h * cbar;
+        bkx += (cg_k * P) * ex + tk * (g0 - P * ex) + (c_k * cbar) * ex - g2 * u.gx - g3 * u.gy;
+        bky += (cg_k * P) * ey + tk * (g1 - P * ey) + (c_k * cbar) * ey - g2 * v.gx - g3 * v.gy;
+        bx += hbar * h.dx + g0 * u.dx + g1 * v.dx;
+        by += hbar * h.dy + g0 * u.dy + g1 * v.dy;
+        // (a state of a finite row never errs; the guards keep a misuse in bounds)
+        if (BK == BK_GRID && Gr.depth && okb)
+            acc_add(acc.h, Gr.depth, (uint32_t)B.nx, bc, B.rxs, B.rys, hbar, g2 * c, g3 * c);
+    }
     if (CK == CK_GRID && okc)
         acc_add_uv(acc.u, acc.v, Gr.u, Gr.v, (uint32_t)C.nx, cc, C.rxs, C.rys, g0, -g2 * kx,
                    -g3 * kx, g1, -g2 * ky, -g3 * ky);

@@ -139,6 +139,14 @@ typedef struct {
     ld *bx, *by, *cx, *cy;        /* axes widened */
 } axes_ld;
 
+/* Rust's saturating `as usize` of a long double (NaN and negatives to 0) */
+static uint64_t sat_usize_ld(ld v)
+{
+    if (!(v > 0.0L)) return 0;
+    if (v >= 18446744073709551616.0L) return UINT64_MAX;
+    return (uint64_t)v;
+}
+
 static int bathy_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld *h, ld *hx, ld *hy,
                     uint64_t *sig)
 {
@@ -155,8 +163,15 @@ static int bathy_ld(const fields_ld *F, const axes_ld *A, ld x, ld y, ld *h, ld 
             *hx = (ld)b->dhdx; *hy = (ld)b->dhdy;
         }
         return OR_OK;
-    case OR_BATHY_ARRAY:
-        return OR_PANIC;          /* not used by the adjoint checks */
+    case OR_BATHY_ARRAY: {    /* array_depth.rs:26-33: the point truncated to indices */
+        const uint64_t xi = sat_usize_ld(x), yi = sat_usize_ld(y);
+        if (xi >= (uint64_t)b->an || yi >= (uint64_t)b->an) {
+            *h = NAN; *hx = NAN; *hy = NAN;
+        } else {
+            *h = (ld)b->arr[xi * (uint64_t)b->an + yi]; *hx = 0.0L; *hy = 0.0L;
+        }
+        return OR_OK;
+    }
     default: break;
     }
     if (isnan(x) || isnan(y)) { *h = NAN; *hx = NAN; *hy = NAN; return OR_OK; }

@@ -143,6 +143,36 @@ def test_ld_traces_end_where_f64_traces_end(oracle_mod):
     assert np.abs(mid[:, :2] - r["end"][:, :2]).max() <= 1e-8
 
 
+@pytest.mark.parametrize("kind", ["slope", "array"])
+def test_ld_traces_analytic_depths(oracle_mod, kind):
+    """The long-double trace over ConstantSlope and ArrayDepth (the array
+    truncates the point to indices, array_depth.rs:26-33, and a point off
+    the array has NaN depth) ends where the f64 oracle's fp64 mode ends: the
+    same row counts, and the end states within the f64 trace's rounding."""
+    O = oracle_mod
+    rng = np.random.default_rng(41)
+    if kind == "slope":
+        ob = O.Bathymetry.slope(h0=15.0, x0=30.0, y0=30.0, dhdx=0.08, dhdy=-0.05)
+    else:
+        ob = O.Bathymetry.array(rng.uniform(10.0, 20.0, (60, 60)))
+    ax = np.linspace(0.0, 59.0, 12)
+    X, Y = np.meshgrid(ax, ax)
+    oc = O.Current.grid(ax, ax, 0.3 * np.sin(Y / 9.0), 0.2 * np.cos(X / 7.0))
+    n = 64
+    th = rng.uniform(0.0, 2.0 * np.pi, n)
+    k = rng.uniform(0.2, 1.0, n)
+    s0 = np.stack([rng.uniform(2.0, 57.0, n), rng.uniform(2.0, 57.0, n), k * np.cos(th),
+                   k * np.sin(th)], 1)
+    r = O.trace_many(ob, oc, *s0.T, 0.0, 12.0, 0.1, prec=O.PREC_FP64_INTERP,
+                     trajectories=False)
+    _, mid, lp, _ = O.fd_end_ld(ob, oc, s0, 0.0, 12.0, 0.1)
+    assert np.array_equal(lp, r["len"])
+    fin = ~np.isnan(r["end"][:, 0])
+    assert fin.sum() >= 32 and (kind == "slope" or (~fin).sum() >= 1)
+    assert np.array_equal(np.isnan(mid), np.isnan(r["end"]))
+    assert np.abs(mid[fin, :2] - r["end"][fin, :2]).max() <= 1e-10
+
+
 def test_ld_central_differences_closed_form(oracle_mod):
     """Straight deep-water rays (h = 2000 m, k >= 0.03: tanh(kh) = 1 in
     long double): x(T) = x0 + T cg kx/|k| with cg = sqrt(g/k)/2."""
