@@ -207,7 +207,7 @@ int mr_rhs_many(const mr_field *bathy, const mr_field *current, int64_t n,
 int mr_checkpoint_count(double t0, double t_end, double dt, int32_t stride, int64_t *count);
 
 /* Bytes of the work buffer mr_adjoint_many needs: 2 * stride * 16 * n_rays
- * doubles (the recomputed states and stage derivatives of two segments). */
+ * doubles (the recomputed states and stage states of two segments). */
 int mr_adjoint_workspace_size(int64_t n_rays, int32_t stride, int64_t *bytes);
 
 /* mr_trace_many in fp64-interpolation mode (whatever mr_options.precision

@@ -17,7 +17,7 @@
 // Rk4::step, k1 .. k4 and (((k1 + 2 k2) + 2 k3) + k4) dt/6), so its
 // gradients are those of the computed trajectories, not of the ODE.  Each
 // lane owns one ray and walks its checkpointed segments backwards: it
-// recomputes the segment's states and stage derivatives from the checkpoint
+// recomputes the segment's states and stage states from the checkpoint
 // with the forward's own step code (bitwise the forward's values) into a
 // per-lane column of the work buffer, then runs the step adjoints in reverse
 // order over them.
@@ -62,8 +62,9 @@ hipError_t adjoint_scratch(void **ptr)
     return hipSuccess;
 }
 
-// One RK4 step in fp64 mode from s: the first three stage derivatives k and
-// the new state o.  The arithmetic is that of trace_kernel's step, and so is
+// One RK4 step in fp64 mode from s: the first three stage derivatives k, the
+// stage states ys (y + k1 dt/2, y + k2 dt/2, y + k3 dt: the states stages 1-3
+// evaluate) and the new state o.  The arithmetic is that of trace_kernel's step, and so is
 // the cell scheme of its shared-cell step: two slots with fixed roles, sa
 // serving st 0 and st 3 (y and y + k3 dt, nearly the next step's start), sb
 // st 1 and st 2 (y + k1 dt/2 and y + k2 dt/2).  Each is refilled while the
@@ -85,7 +86,8 @@ __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs
                                            const Grid64View &C, const Grid64Ptrs &CP,
                                            const Fields64 &F, const FastConsts &K, Slots64 &sa,
                                            Slots64 &sb, double dt, double half, double sixth,
-                                           const double s[4], double k[3][4], double o[4])
+                                           const double s[4], double k[3][4],
+                                           double ys[3][4], double o[4])
 {
     double sx = s[0], sy = s[1], skx = s[2], sky = s[3];
     double ax = 0.0, ay = 0.0, akx = 0.0, aky = 0.0;
@@ -117,13 +119,16 @@ __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs
         }
         sx = s[0] + dx * hs; sy = s[1] + dy * hs;
         skx = s[2] + dkx * hs; sky = s[3] + dky * hs;
+        if (st < 3) {
+            ys[st][0] = sx; ys[st][1] = sy; ys[st][2] = skx; ys[st][3] = sky;
+        }
     }
     o[0] = s[0] + ax * sixth; o[1] = s[1] + ay * sixth;
     o[2] = s[2] + akx * sixth; o[3] = s[3] + aky * sixth;
 }
 
 // stores a recomputing wave issues after each step: its state and three
-// stage derivatives
+// stage states
 constexpr int RECOMPUTE_STORES = 16;
 
 // ---------------------------------------------------------------------------
@@ -182,11 +187,11 @@ __global__ void __launch_bounds__(256) trace64_kernel(Trace64Args a)
         }
         double o[4] = {MR_NAN, MR_NAN, MR_NAN, MR_NAN};
         if (alive) {
-            double k[3][4], n[4];
+            double k[3][4], ys[3][4], n[4];
             // the row stores of the step before are the last operations
             // issued before this step's first refill
             rk4_step64<BK, CK, MATH, OUT == 1 ? 4 : 0>(B, a.bp, C, a.cp, a.f, K, sa, sb, dt,
-                                                       half, sixth, s, k, n);
+                                                       half, sixth, s, k, ys, n);
             // the MR_SOLOUT_NEXT_DERIV test applies from the second step on
             // (solout is asked only after a step)
             if (a.solout == 1 && s_i > 1 && all_nan4(k[0][0], k[0][1], k[0][2], k[0][3])) {
@@ -245,7 +250,7 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
                                                  const Grads64 &Gr, Accs64 &acc,
                                                  const FastConsts &K, double dt, double half,
                                                  double sixth, const double s[4],
-                                                 const double k[3][4], double lam[4])
+                                                 const double ys[3][4], double lam[4])
 {
     double kb[4][4];
 #pragma unroll
@@ -258,9 +263,8 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
     Y[0][0] = s[0]; Y[0][1] = s[1];
 #pragma unroll
     for (int st = 1; st < 4; ++st) {
-        const double hs = st < 3 ? half : dt;
-        Y[st][0] = s[0] + k[st - 1][0] * hs;
-        Y[st][1] = s[1] + k[st - 1][1] * hs;
+        Y[st][0] = ys[st - 1][0];
+        Y[st][1] = ys[st - 1][1];
     }
     Slots64 sq = sn;
 #pragma unroll
@@ -269,9 +273,8 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
         if (st == 0) {
             Yk[0] = s[2]; Yk[1] = s[3];
         } else {
-            const double hs = st < 3 ? half : dt;
-            Yk[0] = s[2] + k[st - 1][2] * hs;
-            Yk[1] = s[3] + k[st - 1][3] * hs;
+            Yk[0] = ys[st - 1][2];
+            Yk[1] = ys[st - 1][3];
         }
         // The stage points pair up: Y2 lies with Y1, and Y0 (this step's
         // start) with the previous step's Y3.  Stages 4 and 2 (st 3, 1) start
@@ -303,7 +306,7 @@ __device__ __forceinline__ void rk4_step_adjoint(const Grid64View &B, const Grid
 // A workgroup is two waves over the same 64 rays, pipelined over the
 // checkpointed segments, latest first: while wave 1 runs the step adjoints
 // of segment c from one half of the work buffer, wave 0 recomputes segment
-// c-1 (states and stage derivatives, from its checkpoint) into the other
+// c-1 (states and stage states, from its checkpoint) into the other
 // half; a barrier separates the rounds.  A segment then costs the longer of
 // the two passes instead of their sum, on SIMDs a one-wave-per-CU launch
 // leaves idle.  Lanes past n_rays stay in the loop, masked, for the
@@ -351,16 +354,16 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
                 load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
                 load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
                 for (int j = 0; j < nseg; ++j) {
-                    double k[3][4], n[4];
+                    double k[3][4], ys[3][4], n[4];
                     rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb,
-                                                               dt, half, sixth, s, k, n);
+                                                               dt, half, sixth, s, k, ys, n);
                     double *w = wk + (int64_t)j * 16 * ld;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         w[q * ld] = s[q];
-                        w[(4 + q) * ld] = k[0][q];
-                        w[(8 + q) * ld] = k[1][q];
-                        w[(12 + q) * ld] = k[2][q];
+                        w[(4 + q) * ld] = ys[0][q];
+                        w[(8 + q) * ld] = ys[1][q];
+                        w[(12 + q) * ld] = ys[2][q];
                         s[q] = n[q];
                     }
                 }
@@ -389,19 +392,19 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
             const int64_t i0 = (int64_t)c * stride;
             const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
             const double *wk = a.work + ((t - 1) & 1) * half_work + r;
-            // each step's state and stage derivatives are loaded one step
+            // each step's state and stage states are loaded one step
             // ahead, while the step before them is swept
             double nxt[16];
 #pragma unroll
             for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
             for (int j = nseg - 1; j >= 0; --j) {
-                double k[3][4], y[4];
+                double ys[3][4], y[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     y[q] = nxt[q];
-                    k[0][q] = nxt[4 + q];
-                    k[1][q] = nxt[8 + q];
-                    k[2][q] = nxt[12 + q];
+                    ys[0][q] = nxt[4 + q];
+                    ys[1][q] = nxt[8 + q];
+                    ys[2][q] = nxt[12 + q];
                 }
                 if (j >= 1) {
                     const double *w = wk + (int64_t)(j - 1) * 16 * ld;
@@ -409,10 +412,9 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
                     for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
-                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, y[0] + k[2][0] * dt,
-                                       y[1] + k[2][1] * dt);
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, ys[2][0], ys[2][1]);
                 rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sn, a.grads, acc, K, dt, half,
-                                         sixth, y, k, lam);
+                                         sixth, y, ys, lam);
                 const int64_t o = (i0 + j) * nr + r;
                 if (a.g_x) lam[0] += a.g_x[o];
                 if (a.g_y) lam[1] += a.g_y[o];
@@ -487,16 +489,16 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
             load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
             load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
             for (int j = 0; j < nseg; ++j) {
-                double k[3][4], n[4];
+                double k[3][4], ys[3][4], n[4];
                 rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb, dt,
-                                                           half, sixth, s, k, n);
+                                                           half, sixth, s, k, ys, n);
                 double *w = wk + (int64_t)j * 16 * ld;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     w[q * ld] = s[q];
-                    w[(4 + q) * ld] = k[0][q];
-                    w[(8 + q) * ld] = k[1][q];
-                    w[(12 + q) * ld] = k[2][q];
+                    w[(4 + q) * ld] = ys[0][q];
+                    w[(8 + q) * ld] = ys[1][q];
+                    w[(12 + q) * ld] = ys[2][q];
                     s[q] = n[q];
                 }
             }
@@ -505,13 +507,13 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
 #pragma unroll
             for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
             for (int j = nseg - 1; j >= 0; --j) {
-                double k[3][4], y[4];
+                double ys[3][4], y[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     y[q] = nxt[q];
-                    k[0][q] = nxt[4 + q];
-                    k[1][q] = nxt[8 + q];
-                    k[2][q] = nxt[12 + q];
+                    ys[0][q] = nxt[4 + q];
+                    ys[1][q] = nxt[8 + q];
+                    ys[2][q] = nxt[12 + q];
                 }
                 if (j >= 1) {
                     const double *w = wk + (int64_t)(j - 1) * 16 * ld;
@@ -519,10 +521,9 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
                     for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
-                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, sa, y[0] + k[2][0] * dt,
-                                       y[1] + k[2][1] * dt);
+                    prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, sa, ys[2][0], ys[2][1]);
                 rk4_step_adjoint<BK, CK>(B, a.bp, C, a.cp, a.f, sa, a.grads, acc, K, dt, half,
-                                         sixth, y, k, lam);
+                                         sixth, y, ys, lam);
                 const int64_t o = (i0 + j) * nr + r;
                 if (a.g_x) lam[0] += a.g_x[o];
                 if (a.g_y) lam[1] += a.g_y[o];
@@ -565,11 +566,15 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
 // 256 registers a wave then gets spill.
 // Whether linearising wave W (NL = 2: 1 or 2; 0: every wave otherwise)
 // uses field f of a step's data in the work buffer: f in [0, 4) is the
-// state y, [4, 12) the stage derivatives k1 and k2, [12, 16) k3.
+// state y, [4, 8), [8, 12) and [12, 16) the stage states of stages 1, 2, 3
+// (wave 1 evaluates stages 2 and 1 and guesses its next cell from y's
+// position; wave 2 evaluates stages 3 and 0).
 template <int W>
 __device__ constexpr bool lin_uses(int f)
 {
-    return f < 4 || (f < 12 ? W != 2 : W != 1);
+    if (W == 1) return f < 2 || (f >= 4 && f < 12);
+    if (W == 2) return f < 4 || f >= 12;
+    return true;
 }
 
 template <int BK, int CK, int MATH, int NL>
@@ -614,16 +619,16 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                         load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
                         load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sb, s[0], s[1]);
                     }
-                    double k[3][4], n[4];
+                    double k[3][4], ys[3][4], n[4];
                     rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa,
-                                                               sb, dt, half, sixth, s, k, n);
+                                                               sb, dt, half, sixth, s, k, ys, n);
                     double *w = a.work + (c & 1) * half_work + r + (int64_t)j * 16 * ld;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         w[q * ld] = s[q];
-                        w[(4 + q) * ld] = k[0][q];
-                        w[(8 + q) * ld] = k[1][q];
-                        w[(12 + q) * ld] = k[2][q];
+                        w[(4 + q) * ld] = ys[0][q];
+                        w[(8 + q) * ld] = ys[1][q];
+                        w[(12 + q) * ld] = ys[2][q];
                         s[q] = n[q];
                     }
                 }
@@ -633,11 +638,9 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
     } else if (wave < chain) {
         // W: the linearising wave with NL = 2 (1 or 2), as a compile-time
         // value, so that each loads and keeps only the step data its stages
-        // use (y with k1 and k2 for wave 1, y with k3 for wave 2); 0 for the
-        // general case
+        // use; 0 for the general case
         auto lin_role = [&](auto wc) {
             constexpr int W = decltype(wc)::value;
-            constexpr bool use_k12 = W != 2, use_k3 = W != 1;
             Slots64 sn;
             double nxt[16];                                // the next step's data
             bool pre = false;
@@ -666,16 +669,17 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                             if (lin_uses<W>(f)) nxt[f] = w[f * ld];
                     }
                     // the four stage points y, y + k1 dt/2, y + k2 dt/2, y + k3 dt
+                    // (the forward's own stage states)
                     double P[4][4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        P[0][q] = d[q];
-                        if (use_k12) {
-                            P[1][q] = d[q] + d[4 + q] * half;
-                            P[2][q] = d[q] + d[8 + q] * half;
-                        }
-                        if (use_k3) P[3][q] = d[q] + d[12 + q] * dt;
-                    }
+                    for (int st = 0; st < 4; ++st)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (lin_uses<W>(4 * st + q)) P[st][q] = d[4 * st + q];
+                    // one step back along k1 from a point p: p - k1 dt, with
+                    // k1 dt = 2 (P[1] - y) (only a guess of where to load)
+                    const double bx = W == 2 ? 0.0 : 2.0 * (d[0] - P[1][0]);
+                    const double by = W == 2 ? 0.0 : 2.0 * (d[1] - P[1][1]);
                     if (NL == 2) {
                         // The stage points pair up: stages 2 and 1 lie together near
                         // y + k dt/2, and stage 0 here lies with stage 3 of the next
@@ -688,8 +692,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                         Slots64 sq = sn;
                         double R[LIN_N];
                         if constexpr (W == 1) {
-                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] - d[4] * dt,
-                                               P[1][1] - d[5] * dt);
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] + bx,
+                                               P[1][1] + by);
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
                                               P[2][2], P[2][3], R);
 #pragma unroll
@@ -741,8 +745,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                                                 : first == 1 ? P[1][0] : P[0][0];
                                 const double fy = first == 3 ? P[3][1] : first == 2 ? P[2][1]
                                                 : first == 1 ? P[1][1] : P[0][1];
-                                gx = fx - d[4] * dt;
-                                gy = fy - d[5] * dt;
+                                gx = fx + bx;
+                                gy = fy + by;
                             }
                             prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, gx, gy);
                             double R[LIN_N];
