@@ -645,6 +645,15 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
             double nxt[16];                                // the next step's data
             bool pre = false;
             for (int rho = 0; rho < rounds; ++rho) {
+                // Everything this wave loaded in the round before (the step
+                // data and the cells for this round) has landed by now; an
+                // explicit wait here says so to the compiler.  Without it the
+                // copy of the cell slot below waits for the loads issued
+                // before it in this round as well (they are conditional, so
+                // the compiler cannot count them): a memory latency a round.
+                // For the same reason the next round's step data is loaded
+                // after everything this round's stage points need.
+                vm_wait_upto<0>();
                 const int qq = rho / stride;
                 const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
                 if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
@@ -654,19 +663,14 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
 #pragma unroll
                         for (int f = 0; f < 16; ++f)
                             if (lin_uses<W>(f)) d[f] = nxt[f];
-                    } else {
+                    } else {                // a segment's first round
                         const double *w = wb + (int64_t)j * 16 * ld;
 #pragma unroll
                         for (int f = 0; f < 16; ++f)
                             if (lin_uses<W>(f)) d[f] = w[f * ld];
-                    }
-                    // start loading the next round's step (same segment)
-                    pre = j >= 1;
-                    if (pre) {
-                        const double *w = wb + (int64_t)(j - 1) * 16 * ld;
-#pragma unroll
-                        for (int f = 0; f < 16; ++f)
-                            if (lin_uses<W>(f)) nxt[f] = w[f * ld];
+                        // (waited for here, so that after the join the
+                        // compiler sees nothing pending on d)
+                        vm_wait_upto<0>();
                     }
                     // the four stage points y, y + k1 dt/2, y + k2 dt/2, y + k3 dt
                     // (the forward's own stage states)
@@ -680,6 +684,23 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     // k1 dt = 2 (P[1] - y) (only a guess of where to load)
                     const double bx = W == 2 ? 0.0 : 2.0 * (d[0] - P[1][0]);
                     const double by = W == 2 ? 0.0 : 2.0 * (d[1] - P[1][1]);
+                    Slots64 sq = sn;
+                    if (NL == 2) {
+                        // the cells of this wave's next point, into sn (see below)
+                        if constexpr (W == 1)
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] + bx,
+                                               P[1][1] + by);
+                        else
+                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
+                    }
+                    // start loading the next round's step (same segment)
+                    pre = j >= 1;
+                    if (pre) {
+                        const double *w = wb + (int64_t)(j - 1) * 16 * ld;
+#pragma unroll
+                        for (int f = 0; f < 16; ++f)
+                            if (lin_uses<W>(f)) nxt[f] = w[f * ld];
+                    }
                     if (NL == 2) {
                         // The stage points pair up: stages 2 and 1 lie together near
                         // y + k dt/2, and stage 0 here lies with stage 3 of the next
@@ -689,11 +710,8 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                         // step back along k1), wave 2 for stage 0 (known), which
                         // it evaluates in the prefetched slot (the lanes that did
                         // not prefetch hold the same cells in both).
-                        Slots64 sq = sn;
                         double R[LIN_N];
                         if constexpr (W == 1) {
-                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[1][0] + bx,
-                                               P[1][1] + by);
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[2][0], P[2][1],
                                               P[2][2], P[2][3], R);
 #pragma unroll
@@ -705,7 +723,6 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                             for (int f = 0; f < LIN_N; ++f)
                                 if (lin_field_used<BK, CK>(f)) ring[rho & 1][1][f][lane] = R[f];
                         } else {
-                            prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sq, sn, P[0][0], P[0][1]);
                             rhs64_lin<BK, CK>(B, a.bp, C, a.cp, a.f, sq, K, P[3][0], P[3][1],
                                               P[3][2], P[3][3], R);
 #pragma unroll
