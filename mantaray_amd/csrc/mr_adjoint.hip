@@ -127,9 +127,42 @@ __device__ __forceinline__ void rk4_step64(const Grid64View &B, const Grid64Ptrs
     o[2] = s[2] + akx * sixth; o[3] = s[3] + aky * sixth;
 }
 
+// The work buffer: two halves of [stride][8][work_ld] pairs of doubles.
+// Step j of a half holds, for ray r, the state and the three stage states as
+// eight 16-byte pairs at (j * 8 + p) * work_ld + r, so that one store or load
+// of a wave moves 64 consecutive pairs (1 KB) and a step takes eight of them.
+__device__ __forceinline__ void work_put(double *half, int64_t ld, int64_t r, int64_t j,
+                                         const double s[4], const double ys[3][4])
+{
+    double2 *w = reinterpret_cast<double2 *>(half) + j * 8 * ld + r;
+    w[0] = make_double2(s[0], s[1]);
+    w[ld] = make_double2(s[2], s[3]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        w[(2 + 2 * i) * ld] = make_double2(ys[i][0], ys[i][1]);
+        w[(3 + 2 * i) * ld] = make_double2(ys[i][2], ys[i][3]);
+    }
+}
+
+// loads the fields f of step j for which use(f) holds (use(2p) == use(2p+1))
+template <class Use>
+__device__ __forceinline__ void work_get(const double *half, int64_t ld, int64_t r, int64_t j,
+                                         double d[16], Use use)
+{
+    const double2 *w = reinterpret_cast<const double2 *>(half) + j * 8 * ld + r;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        if (use(2 * p)) {
+            const double2 v = w[p * ld];
+            d[2 * p] = v.x;
+            d[2 * p + 1] = v.y;
+        }
+    }
+}
+
 // stores a recomputing wave issues after each step: its state and three
-// stage states
-constexpr int RECOMPUTE_STORES = 16;
+// stage states, in pairs
+constexpr int RECOMPUTE_STORES = 8;
 
 // ---------------------------------------------------------------------------
 // fp64 trace: the loop of trace_kernel (mr_kernels.hip) with rhs64 and the
@@ -348,7 +381,7 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
             if (c >= 0 && c <= cmax_r) {
                 const int64_t i0 = (int64_t)c * stride;
                 const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
-                double *wk = a.work + (t & 1) * half_work + r;
+                double *wk = a.work + (t & 1) * half_work;
                 const double *cp = a.ckpt + ((int64_t)c * 4) * nr + r;
                 double s[4] = {cp[0], cp[nr], cp[2 * nr], cp[3 * nr]};
                 load_guess64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, s[0], s[1]);
@@ -357,15 +390,9 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
                     double k[3][4], ys[3][4], n[4];
                     rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb,
                                                                dt, half, sixth, s, k, ys, n);
-                    double *w = wk + (int64_t)j * 16 * ld;
+                    work_put(wk, ld, r, j, s, ys);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        w[q * ld] = s[q];
-                        w[(4 + q) * ld] = ys[0][q];
-                        w[(8 + q) * ld] = ys[1][q];
-                        w[(12 + q) * ld] = ys[2][q];
-                        s[q] = n[q];
-                    }
+                    for (int q = 0; q < 4; ++q) s[q] = n[q];
                 }
             }
             __syncthreads();
@@ -391,12 +418,11 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
         if (t >= 1 && c <= cmax_r) {
             const int64_t i0 = (int64_t)c * stride;
             const int nseg = (int)__builtin_elementwise_min((int64_t)stride, m - i0);
-            const double *wk = a.work + ((t - 1) & 1) * half_work + r;
+            const double *wk = a.work + ((t - 1) & 1) * half_work;
             // each step's state and stage states are loaded one step
             // ahead, while the step before them is swept
             double nxt[16];
-#pragma unroll
-            for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
+            work_get(wk, ld, r, nseg - 1, nxt, [](int) { return true; });
             for (int j = nseg - 1; j >= 0; --j) {
                 double ys[3][4], y[4];
 #pragma unroll
@@ -407,9 +433,7 @@ __global__ void __launch_bounds__(ADJ_BLOCK) adjoint_kernel(AdjointArgs a)
                     ys[2][q] = nxt[12 + q];
                 }
                 if (j >= 1) {
-                    const double *w = wk + (int64_t)(j - 1) * 16 * ld;
-#pragma unroll
-                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
+                    work_get(wk, ld, r, j - 1, nxt, [](int) { return true; });
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
                     prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sn, sn, ys[2][0], ys[2][1]);
@@ -478,7 +502,7 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
     // loaded whole at each segment's start)
     Slots64 sa, sb;
     Accs64 acc;
-    double *wk = a.work + r;                           // the first half of the buffer
+    double *wk = a.work;                               // the first half of the buffer
     for (int c = cmax; c >= 0; --c) {
         if (c <= cmax_r) {
             const int64_t i0 = (int64_t)c * stride;
@@ -492,20 +516,13 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
                 double k[3][4], ys[3][4], n[4];
                 rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa, sb, dt,
                                                            half, sixth, s, k, ys, n);
-                double *w = wk + (int64_t)j * 16 * ld;
+                work_put(wk, ld, r, j, s, ys);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    w[q * ld] = s[q];
-                    w[(4 + q) * ld] = ys[0][q];
-                    w[(8 + q) * ld] = ys[1][q];
-                    w[(12 + q) * ld] = ys[2][q];
-                    s[q] = n[q];
-                }
+                for (int q = 0; q < 4; ++q) s[q] = n[q];
             }
             // and sweep it back, each step's data loaded one step ahead
             double nxt[16];
-#pragma unroll
-            for (int f = 0; f < 16; ++f) nxt[f] = wk[(int64_t)(nseg - 1) * 16 * ld + f * ld];
+            work_get(wk, ld, r, nseg - 1, nxt, [](int) { return true; });
             for (int j = nseg - 1; j >= 0; --j) {
                 double ys[3][4], y[4];
 #pragma unroll
@@ -516,9 +533,7 @@ __global__ void __launch_bounds__(64) adjoint_seq_kernel(AdjointArgs a)
                     ys[2][q] = nxt[12 + q];
                 }
                 if (j >= 1) {
-                    const double *w = wk + (int64_t)(j - 1) * 16 * ld;
-#pragma unroll
-                    for (int f = 0; f < 16; ++f) nxt[f] = w[f * ld];
+                    work_get(wk, ld, r, j - 1, nxt, [](int) { return true; });
                 }
                 if (j == nseg - 1)     // the sweep starts at the segment's last stage point
                     prefetch64<BK, CK>(B, a.bp, C, a.cp, a.f, sa, sa, ys[2][0], ys[2][1]);
@@ -622,15 +637,9 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     double k[3][4], ys[3][4], n[4];
                     rk4_step64<BK, CK, MATH, RECOMPUTE_STORES>(B, a.bp, C, a.cp, a.f, K, sa,
                                                                sb, dt, half, sixth, s, k, ys, n);
-                    double *w = a.work + (c & 1) * half_work + r + (int64_t)j * 16 * ld;
+                    work_put(a.work + (c & 1) * half_work, ld, r, j, s, ys);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        w[q * ld] = s[q];
-                        w[(4 + q) * ld] = ys[0][q];
-                        w[(8 + q) * ld] = ys[1][q];
-                        w[(12 + q) * ld] = ys[2][q];
-                        s[q] = n[q];
-                    }
+                    for (int q = 0; q < 4; ++q) s[q] = n[q];
                 }
             }
             __syncthreads();
@@ -657,17 +666,14 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                 const int qq = rho / stride;
                 const int c = cmax - qq + 1, j = stride - 1 - rho % stride;
                 if (qq >= 1 && c >= 0 && c <= cmax_r && j < m - (int64_t)c * stride) {
-                    const double *wb = a.work + (c & 1) * half_work + r;
+                    const double *wb = a.work + (c & 1) * half_work;
                     double d[16];
                     if (pre) {
 #pragma unroll
                         for (int f = 0; f < 16; ++f)
                             if (lin_uses<W>(f)) d[f] = nxt[f];
                     } else {                // a segment's first round
-                        const double *w = wb + (int64_t)j * 16 * ld;
-#pragma unroll
-                        for (int f = 0; f < 16; ++f)
-                            if (lin_uses<W>(f)) d[f] = w[f * ld];
+                        work_get(wb, ld, r, j, d, [](int f) { return lin_uses<W>(f); });
                         // (waited for here, so that after the join the
                         // compiler sees nothing pending on d)
                         vm_wait_upto<0>();
@@ -696,10 +702,7 @@ __global__ void __launch_bounds__(64 * (NL + 2)) adjoint_lin_kernel(AdjointArgs 
                     // start loading the next round's step (same segment)
                     pre = j >= 1;
                     if (pre) {
-                        const double *w = wb + (int64_t)(j - 1) * 16 * ld;
-#pragma unroll
-                        for (int f = 0; f < 16; ++f)
-                            if (lin_uses<W>(f)) nxt[f] = w[f * ld];
+                        work_get(wb, ld, r, j - 1, nxt, [](int f) { return lin_uses<W>(f); });
                     }
                     if (NL == 2) {
                         // The stage points pair up: stages 2 and 1 lie together near

@@ -126,7 +126,7 @@ struct AdjointArgs {
     const double *end;                 // last row of each ray (4 per ray)
     const double *g_end;               // cotangent of the last row (4 per ray), may be null
     const double *g_x, *g_y, *g_kx, *g_ky;   // cotangents of the rows, step-major, may be null
-    double *work;                      // [2][stride][16][work_ld]
+    double *work;                      // [2][stride][8][work_ld] pairs of doubles
     int64_t work_ld;                   // >= blocks * block threads
     double *g_x0, *g_y0, *g_kx0, *g_ky0;     // may be null
     Grads64 grads;                     // gradient grids, cleared by the caller
